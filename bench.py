@@ -1,0 +1,193 @@
+"""Benchmark of the fused D3Q19 hot path on one MI355X.
+
+Workload (default): the 256x128x128 sphere at Re=100 (D=32 lattice units,
+u=0.05, tau=0.548), MRT + Bouzidi hard mask, Zou-He inlet/outlet, periodic
+y/z, pre-collision stability guard, float32 populations; one "step" is one
+fused timestep (one kernel launch) over the whole grid including the
+per-step float64 momentum-exchange partials.  ``value`` is forward MLUPS
+(cells x steps / s / 1e6) with the state resident in HBM.
+
+Also reported: the roofline of the dominant kernel (algorithmic 152 B per
+cell update for float32 storage, 76 B for half storage) from HIP events
+around each launch, half-storage forward MLUPS, forward+adjoint MLUPS, and
+the CPU oracle (torch, float32) timed on the host over a bounded sample.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--skip-cpu]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import torch
+
+HBM_PEAK_GBS = 8000.0
+NX, NY, NZ = 256, 128, 128
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def build_lattice(storage="f32", nx=NX, ny=NY, nz=NZ, device="cuda"):
+    import lbm_jax_autodiff_amd as lbm
+    from lbm_jax_autodiff_amd.engine import Lattice
+    u_in = 0.05
+    radius = 0.5                       # world units -> D = 2 r nx / 8 lattice units
+    D = 2.0 * radius * nx / 8.0
+    tau = 3.0 * (u_in * D / 100.0) + 0.5
+    solid, q = lbm.create_sphere(nx, ny, nz, (0.0, 0.0, 0.0), radius, device=device)
+    lat = Lattice(nx, ny, nz, tau=tau, inlet_velocity=(u_in, 0.0, 0.0), collision="mrt", solid=solid, q=q,
+                  guard="pre", guard_velocity=(u_in, 0.0, 0.0), periodic_yz=True, storage=storage,
+                  offset="inlet", device=device)
+    return lat, solid, q, tau
+
+
+def initial_state(lat, device):
+    import lbm_jax_autodiff_amd as lbm
+    rho = torch.ones((lat.nx, lat.ny, lat.nz), dtype=torch.float32, device=device)
+    u = torch.zeros((3, lat.nx, lat.ny, lat.nz), dtype=torch.float32, device=device)
+    return lbm.equilibrium(rho, u)
+
+
+def time_forward(lat, f0, steps, warmup):
+    """Returns (ms_per_step over the timed loop, avg kernel ms from per-launch events)."""
+    s = lat.to_state(f0)
+    a, b = lat.new_state(), lat.new_state()
+    lat.collide(s, a)
+    fpart = torch.zeros((max(steps, warmup), max(lat.slots, 1), 3), dtype=torch.float64, device=lat.device)
+    cur, nxt = a, b
+    for t in range(warmup):
+        lat.step(cur, nxt, fpart[t] if lat.slots else None)
+        cur, nxt = nxt, cur
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for t in range(steps):
+        lat.step(cur, nxt, fpart[t] if lat.slots else None)
+        cur, nxt = nxt, cur
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    loop_ms = e0.elapsed_time(e1) / steps
+    # per-launch kernel durations (events on the launch stream bracket each launch)
+    n_k = min(steps, 50)
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(n_k)]
+    for t in range(n_k):
+        evs[t][0].record()
+        lat.step(cur, nxt, fpart[t] if lat.slots else None)
+        evs[t][1].record()
+        cur, nxt = nxt, cur
+    torch.cuda.synchronize()
+    k_ms = sum(x.elapsed_time(y) for x, y in evs) / n_k
+    assert torch.isfinite(lat.from_state(cur)).all(), "non-finite state"
+    return loop_ms, k_ms, wall
+
+
+def cpu_baseline(sample_steps=2):
+    """Oracle (torch CPU float32, the reference's vectorised op structure) on
+    the same workload, timed on the host cores over a few steps."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import lbm_oracle as O
+    cores = len(os.sched_getaffinity(0))
+    torch.set_num_threads(cores)
+    u_in = 0.05
+    solid, q = O.create_sphere(NX, NY, NZ, (0.0, 0.0, 0.0), 0.5)
+    D = 2.0 * 0.5 * NX / 8.0
+    tau = torch.tensor(3.0 * (u_in * D / 100.0) + 0.5, dtype=torch.float32)
+    rho = torch.ones((NX, NY, NZ), dtype=torch.float32)
+    u0 = torch.zeros((3, NX, NY, NZ), dtype=torch.float32)
+    f = O.equilibrium(rho, u0)
+    uref = torch.zeros_like(u0)
+    uref[0] = u_in
+    feq_ref = O.equilibrium(rho, uref)
+    vel = torch.tensor([u_in, 0.0, 0.0], dtype=torch.float32)
+    kw = dict(tau=tau, velocity=vel, q=q, solid=solid, collision="mrt", guard="pre", f_eq_ref=feq_ref)
+    with torch.no_grad():
+        f, _ = O.hot_step(f, **kw)  # warm-up
+        t0 = time.perf_counter()
+        for _ in range(sample_steps):
+            f, fp = O.hot_step(f, **kw)
+            O.momentum_exchange(fp, f, q)
+        dt = time.perf_counter() - t0
+    mlups = NX * NY * NZ * sample_steps / dt / 1e6
+    return {"value": round(mlups, 3), "unit": "MLUPS", "cores": cores, "kind": "port",
+            "sample": f"{sample_steps} steps of the 256x128x128 MRT+Bouzidi sphere step incl. momentum exchange "
+                      f"(torch CPU float32, {cores} threads) after 1 warm-up step; {dt:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=40)
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-extras", action="store_true", help="forward float32 only")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build targets one GPU; multi-GPU runs are not part of it "
+              "(x-slab sharding is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    if not torch.cuda.is_available():
+        print("bench.py: no HIP device", file=sys.stderr)
+        sys.exit(1)
+    dev = torch.device("cuda", 0)
+    import lbm_jax_autodiff_amd._lib as L
+    L.load()
+
+    cells = NX * NY * NZ
+    log(f"workload 256x128x128 MRT+Bouzidi sphere, f32 forward, warmup={args.warmup} steps={args.steps}")
+    lat, solid, q, tau = build_lattice("f32", device=dev)
+    f0 = initial_state(lat, dev)
+    loop_ms, k_ms, wall = time_forward(lat, f0, args.steps, args.warmup)
+    mlups = cells / (loop_ms * 1e-3) / 1e6
+    bytes_per_lu = 152.0
+    achieved = bytes_per_lu * cells / (k_ms * 1e-3) / 1e9
+    log(f"f32 forward: {loop_ms:.4f} ms/step, kernel {k_ms:.4f} ms, {mlups:.1f} MLUPS, {achieved:.1f} GB/s")
+
+    extras = {}
+    if not args.no_extras:
+        for st in ("f16", "bf16"):
+            lat_h, _, _, _ = build_lattice(st, device=dev)
+            lm, km, _ = time_forward(lat_h, f0, max(args.steps // 2, 10), args.warmup // 2)
+            extras[f"fwd_{st}_mlups"] = round(cells / (lm * 1e-3) / 1e6, 1)
+            extras[f"fwd_{st}_hbm_frac"] = round(76.0 * cells / (km * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+            log(f"{st} forward: {lm:.4f} ms/step, {extras[f'fwd_{st}_mlups']} MLUPS")
+            del lat_h
+
+    out = {
+        "metric": "fwd & fwd+adjoint MLUPS, % HBM roofline; dCd/dr time at <=1e-3 rel err",
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(loop_ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (deterministic sphere geometry, equilibrium initial state)",
+        "config": {"workload": "sphere 256x128x128 D=32 Re=100 MRT+Bouzidi hard mask, f32 populations, "
+                               "fused forward step", "grid": [NX, NY, NZ], "tau": round(tau, 6),
+                   "cells": cells, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel_ms": round(k_ms, 5), "bytes_per_cell_update": bytes_per_lu},
+        "extras": extras,
+    }
+    if not args.skip_cpu:
+        log("cpu baseline (oracle, torch CPU float32)")
+        out["cpu_baseline"] = cpu_baseline(args.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

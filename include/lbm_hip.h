@@ -1,0 +1,258 @@
+/*
+ * lbm_hip.h -- C ABI of liblbm_hip.so, the gfx950 (MI355X) implementation of
+ * the jax_lbm per-timestep hot path (MarcosAsh/LBM_JAX_Autodiff).
+ *
+ * The reference has no FFI: its boundary is the Python surface re-exported at
+ * jax_lbm/__init__.py:23-60.  Each entry point below replaces the reference
+ * function named in its comment; the Python package lbm_jax_autodiff_amd binds
+ * them with ctypes and keeps the reference's names and signatures.
+ *
+ * Conventions
+ *  - All pointers are device pointers unless marked "host".
+ *  - Fields use "lattice storage": x fastest.  Population i of cell (x,y,z) is
+ *    element i*N + (z*ny + y)*nx + x with N = nx*ny*nz; scalar fields drop i;
+ *    vector fields (velocity) put the component first.
+ *  - dtype selects the element type: LBM_F32 float, LBM_F64 double, and for the
+ *    fused step also LBM_F16DEV / LBM_BF16DEV (half-precision deviations
+ *    f_i - off_i, see lbm_step_desc.offset).
+ *  - Every call is asynchronous on `stream` (a hipStream_t; 0 = null stream),
+ *    allocates nothing, and returns 0 or an LBM_E_* / hipError_t code.
+ *  - Calls are reentrant; there is no global mutable state.
+ */
+#ifndef LBM_HIP_H_
+#define LBM_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* lbm_stream_t; /* hipStream_t */
+
+enum {
+  LBM_F32 = 0,
+  LBM_F64 = 1,
+  LBM_F16DEV = 2,
+  LBM_BF16DEV = 3
+};
+
+enum {
+  LBM_OK = 0,
+  LBM_E_ARG = 1001,     /* bad size, null pointer or unsupported flag */
+  LBM_E_DTYPE = 1002,   /* dtype not supported by this entry point */
+  LBM_E_LAUNCH = 1003   /* kernel launch failed (hipGetLastError != 0) */
+};
+
+/* Library version and a probe that the gfx950 code object is loadable. */
+int lbm_version(void);
+int lbm_device_check(void);
+
+/* ---- Equilibrium and moments (jax_lbm/core/equilibrium.py) ---------------- */
+
+/* f_eq = w rho (1 + 3 e.u + 4.5 (e.u)^2 - 1.5 u^2).  Replaces equilibrium()
+ * at jax_lbm/core/equilibrium.py:90-151.  rho: N, u: 3N, f_out: 19N. */
+int lbm_equilibrium(int dtype, const void* rho, const void* u, void* f_out,
+                    int nx, int ny, int nz, lbm_stream_t stream);
+
+/* rho = sum_i f_i.  Replaces compute_density() at equilibrium.py:22-48. */
+int lbm_density(int dtype, const void* f, void* rho_out, int nx, int ny, int nz,
+                lbm_stream_t stream);
+
+/* u = sum_i f_i e_i / rho.  Replaces compute_velocity() at equilibrium.py:51-87. */
+int lbm_velocity(int dtype, const void* f, const void* rho, void* u_out,
+                 int nx, int ny, int nz, lbm_stream_t stream);
+
+/* ---- Collision (jax_lbm/core/collision.py) -------------------------------- */
+
+/* BGK.  tau_field (N values) overrides tau when non-null.  Replaces bgk() at
+ * jax_lbm/core/collision.py:46-100. */
+int lbm_bgk(int dtype, const void* f, double tau, const void* tau_field, void* f_out,
+            int nx, int ny, int nz, lbm_stream_t stream);
+
+/* MRT (d'Humieres basis, omega in the stress slots).  Replaces mrt() at
+ * collision.py:103-184 with mrt_forward/mrt_inverse/mrt_equilibrium_moments
+ * at jax_lbm/core/lattice.py:132-391. */
+int lbm_mrt(int dtype, const void* f, double tau, const void* tau_field, void* f_out,
+            int nx, int ny, int nz, lbm_stream_t stream);
+
+/* tau_eff = (tau + sqrt(tau^2 + 18 cs^2 |Pi_neq| / (rho + 1e-10))) / 2.
+ * Replaces smagorinsky_tau() at collision.py:187-270. */
+int lbm_smagorinsky_tau(int dtype, const void* f, double tau, double cs, void* tau_out,
+                        int nx, int ny, int nz, lbm_stream_t stream);
+
+/* (1 - phi) f + phi f[opp].  Replaces soft_bounce_back() at
+ * jax_lbm/geometry/smooth.py:118-155. */
+int lbm_soft_bounce_back(int dtype, const void* f, const void* porosity, void* f_out,
+                         int nx, int ny, int nz, lbm_stream_t stream);
+
+/* f_out = f[opp] where solid >= 1, else f (solid: int32 N).  Replaces the hard
+ * solid bounce of jax_lbm/state.py:146-149. */
+int lbm_hard_bounce(int dtype, const void* f, const int32_t* solid, void* f_out,
+                    int nx, int ny, int nz, lbm_stream_t stream);
+
+/* ---- Streaming (jax_lbm/core/streaming.py) -------------------------------- */
+
+/* Pull streaming with Bouzidi links from a dense q (19N, same dtype; q<0 = no
+ * wall).  Replaces stream() at jax_lbm/core/streaming.py:40-185. */
+int lbm_stream(int dtype, const void* f, const void* q, int periodic_yz, void* f_out,
+               int nx, int ny, int nz, lbm_stream_t stream);
+
+/* ---- Zou-He boundaries (jax_lbm/core/boundary.py), in place on f ---------- */
+
+/* Velocity inlet on the x = 0 plane.  has_rho_target != 0 uses rho_target.
+ * Replaces zou_he_inlet() at jax_lbm/core/boundary.py:35-124. */
+int lbm_zou_he_inlet(int dtype, void* f, double ux, double uy, double uz,
+                     int has_rho_target, double rho_target,
+                     int nx, int ny, int nz, lbm_stream_t stream);
+
+/* Pressure outlet on the x = nx-1 plane.  Replaces zou_he_outlet() at
+ * boundary.py:127-200. */
+int lbm_zou_he_outlet(int dtype, void* f, double rho_out, int nx, int ny, int nz,
+                      lbm_stream_t stream);
+
+/* ---- Forces (jax_lbm/core/forces.py) -------------------------------------- */
+
+/* Bytes of device workspace lbm_momentum_exchange needs. */
+size_t lbm_reduce_workspace_bytes(int nx, int ny, int nz);
+
+/* out[0:3] = total force, out[3:6] = pressure part (when want_pressure), both
+ * accumulated in float64 with a fixed reduction order.  Replaces
+ * momentum_exchange() (forces.py:37-120) and momentum_exchange_decomposed()
+ * (forces.py:123-196).  q is dense (19N, same dtype as f). */
+int lbm_momentum_exchange(int dtype, const void* f_post, const void* f_streamed,
+                          const void* q, int want_pressure, double* out, void* workspace,
+                          int nx, int ny, int nz, lbm_stream_t stream);
+
+/* out[0] = sum_k a_k b_k over n elements in float64 with a fixed reduction
+ * order; workspace as for lbm_momentum_exchange.  Chains per-element adjoints
+ * with geometry derivatives (e.g. dL/dq . dq/dr). */
+int lbm_dot(int a_dtype, const void* a, int b_dtype, const void* b, size_t n,
+            double* out, void* workspace, lbm_stream_t stream);
+
+/* Number of columns along `axis` holding a body cell (solid == 1), written as
+ * one float to out.  `counter` is one device uint32 of scratch.  Replaces
+ * projected_area() at forces.py:199-234. */
+int lbm_projected_area(const int32_t* solid, int axis, float* out, uint32_t* counter,
+                       int nx, int ny, int nz, lbm_stream_t stream);
+
+/* cd[t] = |F_t,x| / (0.5 rho U^2 A + 1e-10) for T force rows (float64).
+ * area is a device float64 scalar.  Replaces drag_coefficient() at
+ * forces.py:237-276. */
+int lbm_drag_coefficient(const double* force, int T, double inlet_velocity,
+                         const double* area, double rho, double* cd,
+                         lbm_stream_t stream);
+
+/* ---- Geometry (jax_lbm/geometry) ----------------------------------------- */
+
+/* Sphere geometry on the world box [-4,4]x[-2,2]x[-2,2].
+ *  mode 0: hard mask dist^2 <= r^2 with ray q  (create_sphere,
+ *          jax_lbm/geometry/primitives.py:56-174; disc floor 0)
+ *  mode 1: porosity = sigmoid(sharpness (r - dist)), solid = porosity > 1/2,
+ *          ray q with sqrt(max(disc, disc_floor))  (smooth_sphere_geometry,
+ *          jax_lbm/geometry/smooth.py:158-248)
+ * coord_dtype (LBM_F32/LBM_F64) is the precision of the coordinate and ray
+ * arithmetic; out_dtype that of porosity/q/dqdr/dphidr.  porosity, dqdr and
+ * dphidr may be null.  dqdr = dq/dr in world units (0 where q is clipped or
+ * invalid); dphidr = d porosity / dr = sharpness phi (1 - phi) (0 in mode 0). */
+int lbm_sphere_geometry(int mode, int coord_dtype, int out_dtype,
+                        double cx, double cy, double cz, double radius, double sharpness,
+                        double disc_floor, double denom_eps,
+                        int32_t* solid, void* porosity, void* q, void* dqdr, void* dphidr,
+                        int nx, int ny, int nz, lbm_stream_t stream);
+
+/* r - |x_w - c| (positive inside), world units.  Replaces
+ * signed_distance_sphere() at jax_lbm/geometry/smooth.py:34-78. */
+int lbm_sphere_sdf(int coord_dtype, int out_dtype, double cx, double cy, double cz, double radius,
+                   void* sdf, int nx, int ny, int nz, lbm_stream_t stream);
+
+/* sigmoid(sharpness * sdf).  Replaces porosity_from_sdf() at
+ * jax_lbm/geometry/smooth.py:81-115. */
+int lbm_porosity_from_sdf(int dtype, const void* sdf, double sharpness, void* out,
+                          int nx, int ny, int nz, lbm_stream_t stream);
+
+/* q = default_q on every fluid cell whose in-bounds neighbour along e_i is a
+ * body cell, else -1.  Replaces compute_q_from_solid() at
+ * jax_lbm/geometry/bouzidi.py:32-106. */
+int lbm_q_from_solid(int dtype, const int32_t* solid, double default_q, void* q,
+                     int nx, int ny, int nz, lbm_stream_t stream);
+
+/* ---- Link list (compiled form of a dense q for the fused step) ------------ */
+
+/* Pass 1: per-256-cell-block count of link cells (cells with any q_j >= 0,
+ * j = 1..18) into block_counts[ceil(N/256)], and their exclusive prefix into
+ * block_offsets (same size) plus the total into total[0].  The host reads
+ * total[0] to size pass 2's outputs. */
+int lbm_links_count(int dtype, const void* q, int32_t* block_counts, int32_t* block_offsets,
+                    int32_t* total, int nx, int ny, int nz, lbm_stream_t stream);
+
+/* Pass 2: link_cell[n] (cell index, ascending), link_q[18][n] (q_j for j=1..18,
+ * -1 = none), link_dq[18][n] (dq/dr, optional), and flags[N] (bit0 solid >= 1,
+ * bit1 link cell; solid may be null). */
+int lbm_links_build(int dtype, const void* q, const void* dqdr, const int32_t* solid,
+                    const int32_t* block_offsets, int n_links, int32_t* link_cell,
+                    double* link_q, double* link_dq, uint8_t* flags,
+                    int nx, int ny, int nz, lbm_stream_t stream);
+
+/* ---- Fused timestep (jax_lbm/state.py:87-171 and the examples' scan bodies) */
+
+enum { LBM_COLL_BGK = 0, LBM_COLL_MRT = 1 };
+enum { LBM_BOUNCE_NONE = 0, LBM_BOUNCE_HARD = 1, LBM_BOUNCE_SOFT = 2 };
+enum { LBM_GUARD_NONE = 0, LBM_GUARD_POST = 1, LBM_GUARD_PRE = 2 };
+enum {
+  LBM_MODE_FULL = 0,     /* P_t -> pull/Bouzidi/Zou-He/[guard] -> S_{t+1} -> collide/bounce -> P_{t+1} */
+  LBM_MODE_COLLIDE = 1,  /* S_t -> [pre guard]/collide/bounce -> P_t   (rollout prologue) */
+  LBM_MODE_STREAM = 2    /* P_t -> pull/Bouzidi/Zou-He/[post guard] -> S_{t+1} (epilogue) */
+};
+
+typedef struct lbm_step_desc {
+  int32_t nx, ny, nz;
+  int32_t storage;       /* LBM_F32 / LBM_F64 / LBM_F16DEV / LBM_BF16DEV */
+  int32_t collision;     /* LBM_COLL_* */
+  int32_t smagorinsky;   /* 0/1: tau_eff from smagorinsky_tau before collision */
+  int32_t bounce;        /* LBM_BOUNCE_* */
+  int32_t guard;         /* LBM_GUARD_* */
+  int32_t periodic_yz;   /* 1 periodic y/z, 0 clamp (streaming.py:134-142) */
+  int32_t n_links;       /* number of link cells in link_cell / link_q */
+  double tau;
+  double smag_cs;
+  double u_in[3];        /* Zou-He inlet velocity */
+  double rho_out;        /* Zou-He outlet density */
+  double u_ref[3];       /* pre-guard reset target eq(1, u_ref) */
+  double offset[19];     /* half-storage offsets off_i (deviation f_i - off_i) */
+  const uint8_t* flags;     /* N: bit0 solid (hard bounce), bit1 link cell */
+  const void* porosity;     /* N, float (double for LBM_F64): soft bounce */
+  const int32_t* link_cell; /* n_links */
+  const double* link_q;     /* 18 x n_links */
+} lbm_step_desc;
+
+/* Number of float64 force partials (3 each) one lbm_step launch writes. */
+int lbm_step_force_slots(const lbm_step_desc* d);
+
+/* One fused timestep (one kernel launch).  src and dst are distinct state
+ * buffers in d->storage format.  force_partial (optional, 3 x
+ * lbm_step_force_slots doubles) receives the momentum-exchange force
+ * sum over links of e_j (P_t(x)_j + S_{t+1}(x)_opp(j)) as per-workgroup float64
+ * partials (modes FULL and STREAM).  Replaces one body of the reference's
+ * scan loops: jax_lbm/state.py:129-171, examples/10_cd_convergence.py:69-83,
+ * examples/11_optimisation_v2.py:152-160, examples/01_channel_flow.py:51-57. */
+int lbm_step(const lbm_step_desc* d, int mode, const void* src, void* dst,
+             double* force_partial, lbm_stream_t stream);
+
+/* force[t][0:3] = fixed-order float64 sum of the `slots` partials of row t,
+ * for T rows.  Completes momentum_exchange (forces.py:37-120) for the fused
+ * path. */
+int lbm_force_reduce(const double* partials, int T, int slots, double* force,
+                     lbm_stream_t stream);
+
+/* Convert between storage formats (e.g. float32 <-> half deviations) for a
+ * full 19N state. */
+int lbm_convert_state(const lbm_step_desc* d, int src_dtype, const void* src,
+                      int dst_dtype, void* dst, lbm_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LBM_HIP_H_ */

@@ -1,0 +1,189 @@
+// Shared device-side definitions for the D3Q19 kernels (gfx950).
+//
+// Lattice storage ("x-fastest SoA"): population i of cell (x, y, z) lives at
+//   f[i * N + (z * ny + y) * nx + x],   N = nx * ny * nz.
+// Scalar fields use the same cell index; vector fields put the component first.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_bf16.h>
+#include <stdint.h>
+
+#include "../../include/lbm_hip.h"
+
+namespace lbm {
+
+constexpr int Q = 19;
+
+// D3Q19 velocity set in the order of jax_lbm/core/lattice.py:29-49: rest, the
+// six faces (+x,-x,+y,-y,+z,-z), then the xy, xz and yz edges.
+constexpr int EX[Q] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+constexpr int EY[Q] = {0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1};
+constexpr int EZ[Q] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1};
+
+__host__ __device__ constexpr int opp(int i) {
+  // the direction with the negated velocity
+  for (int j = 0; j < Q; ++j)
+    if (EX[j] == -EX[i] && EY[j] == -EY[i] && EZ[j] == -EZ[i]) return j;
+  return -1;
+}
+
+__host__ __device__ constexpr int c2(int i) { return EX[i] * EX[i] + EY[i] * EY[i] + EZ[i] * EZ[i]; }
+
+// Weights rounded to float32, as the reference keeps them (lattice.py:53-60)
+// even in its float64 runs.
+__host__ __device__ constexpr float weight(int i) {
+  return i == 0 ? 1.0f / 3.0f : (c2(i) == 1 ? 1.0f / 18.0f : 1.0f / 36.0f);
+}
+
+// d'Humieres moment matrix, row k evaluated from its defining polynomial of
+// the lattice velocity c_i (row order of lattice.py:140-145).  All entries are
+// small integers, so fully unrolled products fold to adds.
+__host__ __device__ constexpr int mrt_m(int k, int i) {
+  const int cx = EX[i], cy = EY[i], cz = EZ[i];
+  const int s = c2(i);
+  switch (k) {
+    case 0: return 1;
+    case 1: return 19 * s - 30;
+    case 2: return (21 * s * s - 53 * s + 24) / 2;
+    case 3: return cx;
+    case 4: return (5 * s - 9) * cx;
+    case 5: return cy;
+    case 6: return (5 * s - 9) * cy;
+    case 7: return cz;
+    case 8: return (5 * s - 9) * cz;
+    case 9: return 3 * cx * cx - s;
+    case 10: return (3 * s - 5) * (3 * cx * cx - s);
+    case 11: return cy * cy - cz * cz;
+    case 12: return (3 * s - 5) * (cy * cy - cz * cz);
+    case 13: return cx * cy;
+    case 14: return cy * cz;
+    case 15: return cx * cz;
+    case 16: return (cy * cy - cz * cz) * cx;
+    case 17: return (cz * cz - cx * cx) * cy;
+    case 18: return (cx * cx - cy * cy) * cz;
+  }
+  return 0;
+}
+
+// ||M_k||^2 = sum_i M_ki^2 (19, 2394, 252, 10, 40, ... 8).
+__host__ __device__ constexpr int mrt_norm(int k) {
+  int acc = 0;
+  for (int i = 0; i < Q; ++i) acc += mrt_m(k, i) * mrt_m(k, i);
+  return acc;
+}
+
+// Fixed relaxation rates (lattice.py:106-126), float32-rounded like the
+// reference.  Slots 9, 11, 13, 14, 15 take omega = 1/tau; 0, 3, 5, 7 are
+// conserved (rate 0).
+__host__ __device__ constexpr float mrt_rate_fixed(int k) {
+  return k == 1 ? 1.19f : (k == 2 || k == 10 || k == 12) ? 1.4f
+         : (k == 4 || k == 6 || k == 8) ? 1.2f : (k >= 16) ? 1.98f : 0.0f;
+}
+__host__ __device__ constexpr bool mrt_is_stress(int k) {
+  return k == 9 || k == 11 || k == 13 || k == 14 || k == 15;
+}
+__host__ __device__ constexpr bool mrt_is_conserved(int k) {
+  return k == 0 || k == 3 || k == 5 || k == 7;
+}
+
+// ---------------------------------------------------------------------------
+// Storage formats.  Compute is float for the float32 / half / bfloat16
+// formats and double for the float64 format.  Half formats hold the deviation
+// f_i - off_i from a per-direction offset (the weights w_i by default).
+// ---------------------------------------------------------------------------
+struct StoreF32 {
+  using T = float;
+  using R = float;
+  static constexpr int tag = LBM_F32;
+  __device__ static R ld(const T* p, size_t k, R) { return __builtin_nontemporal_load(p + k); }
+  __device__ static R ld_c(const T* p, size_t k, R) { return p[k]; }
+  __device__ static void st(T* p, size_t k, R v, R) { __builtin_nontemporal_store(v, p + k); }
+};
+struct StoreF64 {
+  using T = double;
+  using R = double;
+  static constexpr int tag = LBM_F64;
+  __device__ static R ld(const T* p, size_t k, R) { return __builtin_nontemporal_load(p + k); }
+  __device__ static R ld_c(const T* p, size_t k, R) { return p[k]; }
+  __device__ static void st(T* p, size_t k, R v, R) { __builtin_nontemporal_store(v, p + k); }
+};
+struct StoreF16 {
+  using T = __half;
+  using R = float;
+  static constexpr int tag = LBM_F16DEV;
+  __device__ static R ld(const T* p, size_t k, R off) {
+    const unsigned short raw = __builtin_nontemporal_load(reinterpret_cast<const unsigned short*>(p) + k);
+    return __half2float(__ushort_as_half(raw)) + off;
+  }
+  __device__ static R ld_c(const T* p, size_t k, R off) { return __half2float(p[k]) + off; }
+  __device__ static void st(T* p, size_t k, R v, R off) {
+    const unsigned short raw = __half_as_ushort(__float2half_rn(v - off));
+    __builtin_nontemporal_store(raw, reinterpret_cast<unsigned short*>(p) + k);
+  }
+};
+struct StoreBF16 {
+  using T = __hip_bfloat16;
+  using R = float;
+  static constexpr int tag = LBM_BF16DEV;
+  __device__ static R ld(const T* p, size_t k, R off) {
+    const unsigned short raw = __builtin_nontemporal_load(reinterpret_cast<const unsigned short*>(p) + k);
+    return __uint_as_float(static_cast<unsigned>(raw) << 16) + off;
+  }
+  __device__ static R ld_c(const T* p, size_t k, R off) {
+    const unsigned short raw = reinterpret_cast<const unsigned short*>(p)[k];
+    return __uint_as_float(static_cast<unsigned>(raw) << 16) + off;
+  }
+  __device__ static void st(T* p, size_t k, R v, R off) {
+    // round to nearest even on the deviation
+    const float d = v - off;
+    unsigned u = __float_as_uint(d);
+    unsigned short raw;
+    if ((u & 0x7f800000u) == 0x7f800000u) {
+      raw = static_cast<unsigned short>((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    } else {
+      u += 0x7fffu + ((u >> 16) & 1u);
+      raw = static_cast<unsigned short>(u >> 16);
+    }
+    __builtin_nontemporal_store(raw, reinterpret_cast<unsigned short*>(p) + k);
+  }
+};
+
+// Fast unsigned division by a run-time constant (round-up multiplier method
+// of Granlund & Montgomery, 1994): q = (mulhi(n, m) + n) >> s with
+// s = ceil(log2 d), m = floor(2^32 (2^s - d) / d) + 1.  Exact for n < 2^31.
+struct FastDiv {
+  uint32_t d, m, s;
+};
+inline FastDiv make_fastdiv(uint32_t d) {
+  FastDiv r{d, 0, 0};
+  uint32_t s = 0;
+  while ((1ull << s) < d) ++s;
+  r.s = s;
+  r.m = static_cast<uint32_t>(((1ull << 32) * ((1ull << s) - d)) / d + 1);
+  return r;
+}
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv& f) {
+  return (__umulhi(n, f.m) + n) >> f.s;
+}
+
+// Map a hardware workgroup id to a logical tile so that consecutive logical
+// tiles land on the same XCD (workgroups are dealt round-robin to the 8 XCDs
+// of an MI355X).  Keeps the halo cache lines of neighbouring tiles in one L2.
+__device__ __forceinline__ uint32_t xcd_swizzle(uint32_t bid, uint32_t nblocks) {
+  constexpr uint32_t NX = 8;
+  const uint32_t per = nblocks / NX;
+  const uint32_t full = per * NX;
+  if (bid >= full) return bid;
+  return (bid % NX) * per + bid / NX;
+}
+
+template <typename R>
+__device__ __forceinline__ R wave_sum(R v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+}  // namespace lbm

@@ -1,0 +1,373 @@
+// Fused D3Q19 timestep for gfx950: one launch per step.
+//
+// The carried state is the post-collision, post-bounce population P_t (the
+// reference's f_post).  One launch computes, for every cell x,
+//   S_{t+1}(x) = guard(ZouHe(pull(P_t) with Bouzidi links))
+//   P_{t+1}(x) = bounce(collide(S_{t+1}(x)))
+// reading each population of P_t once (pull streaming is a bijection) and
+// writing P_{t+1} once: 2 x 19 values per cell.
+//
+// Workgroups [0, nlb) own the sparse link cells (fluid cells with a Bouzidi
+// link, listed in link_cell/link_q) and also produce the float64
+// momentum-exchange partials; workgroups [nlb, nlb + ndb) sweep the dense grid
+// and skip link cells.  Dense workgroups are re-mapped so that consecutive
+// x-rows share an XCD (and its L2).
+#include "lbm_physics.hpp"
+
+namespace lbm {
+
+constexpr int kStepBlock = 256;
+
+template <typename R>
+struct StepArgs {
+  int nx, ny, nz;
+  uint32_t N;
+  FastDiv dnx, dny;
+  int coll, smag, bounce, guard, periodic, n_links, nlb, ndb;
+  R tau, cs;
+  R uin[3];
+  R rho_out;
+  R uref[3];
+  R off[Q];
+  const uint8_t* flags;
+  const R* phi;
+  const int32_t* link_cell;
+  const double* link_q;
+};
+
+template <typename R>
+__device__ __forceinline__ void cell_coords(const StepArgs<R>& a, uint32_t c, int& x, int& y, int& z) {
+  const uint32_t yz = fdiv(c, a.dnx);
+  x = int(c - yz * uint32_t(a.nx));
+  const uint32_t zz = fdiv(yz, a.dny);
+  y = int(yz - zz * uint32_t(a.ny));
+  z = int(zz);
+}
+
+// Source cell of the normal pull for direction i (streaming.py:113-142);
+// returns c itself where the reference keeps the cell's own value.
+template <typename R>
+__device__ __forceinline__ uint32_t pull_src(const StepArgs<R>& a, int i, uint32_t c, int x, int y, int z) {
+  const int ex = EX[i], ey = EY[i], ez = EZ[i];
+  if (ex == 1 && x == 0) return c;
+  if (ex == -1 && x == a.nx - 1) return c;
+  int sy = y - ey, sz = z - ez;
+  if (a.periodic) {
+    if (sy < 0) sy += a.ny;
+    if (sy >= a.ny) sy -= a.ny;
+    if (sz < 0) sz += a.nz;
+    if (sz >= a.nz) sz -= a.nz;
+  } else {
+    if (sy < 0 || sy >= a.ny || sz < 0 || sz >= a.nz) return c;
+  }
+  return (uint32_t(sz) * uint32_t(a.ny) + uint32_t(sy)) * uint32_t(a.nx) + uint32_t(x - ex);
+}
+
+// Cell x + e_i wrapped periodically in all three axes (streaming.py:166-172).
+template <typename R>
+__device__ __forceinline__ uint32_t wrap_next(const StepArgs<R>& a, int i, int x, int y, int z) {
+  int fx = x + EX[i], fy = y + EY[i], fz = z + EZ[i];
+  fx = fx < 0 ? fx + a.nx : (fx >= a.nx ? fx - a.nx : fx);
+  fy = fy < 0 ? fy + a.ny : (fy >= a.ny ? fy - a.ny : fy);
+  fz = fz < 0 ? fz + a.nz : (fz >= a.nz ? fz - a.nz : fz);
+  return (uint32_t(fz) * uint32_t(a.ny) + uint32_t(fy)) * uint32_t(a.nx) + uint32_t(fx);
+}
+
+// Zou-He planes and the post-collision guard on a pulled S (state.py:152-169).
+template <typename R>
+__device__ __forceinline__ void boundaries(const StepArgs<R>& a, R* s, int x) {
+  if (x == 0) zou_he_in(s, a.uin[0], a.uin[1], a.uin[2], false, R(0));
+  if (x == a.nx - 1) zou_he_out(s, a.rho_out);
+  if (a.guard == LBM_GUARD_POST && guard_bad(s)) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) s[i] = R(weight(i));
+  }
+}
+
+// [pre guard] -> [Smagorinsky tau] -> BGK|MRT -> hard|soft bounce
+// (examples/10:71-78, state.py:133-149, smooth.py:151-155).
+template <typename R, int COLL>
+__device__ __forceinline__ void collide_cell(const StepArgs<R>& a, R* s, uint32_t c) {
+  if (a.guard == LBM_GUARD_PRE && guard_bad(s)) {
+    const R uu = a.uref[0] * a.uref[0] + a.uref[1] * a.uref[1] + a.uref[2] * a.uref[2];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) s[i] = feq(i, R(1), a.uref[0], a.uref[1], a.uref[2], uu);
+  }
+  R tau = a.tau;
+  if (a.smag) tau = smag_tau(s, a.tau, a.cs);
+  const R omega = R(1) / tau;
+  if (COLL == LBM_COLL_MRT) collide_mrt(s, omega);
+  else collide_bgk(s, omega);
+  if (a.bounce == LBM_BOUNCE_HARD) {
+    if (a.flags[c] & 1) {
+#pragma unroll
+      for (int i = 1; i < Q; ++i) {
+        if (opp(i) > i) {
+          const R t = s[i];
+          s[i] = s[opp(i)];
+          s[opp(i)] = t;
+        }
+      }
+    }
+  } else if (a.bounce == LBM_BOUNCE_SOFT) {
+    const R p = a.phi[c];
+    R t[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) t[i] = s[i];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) s[i] = (R(1) - p) * t[i] + p * t[opp(i)];
+  }
+}
+
+template <class S, int COLL, int MODE>
+__global__ __launch_bounds__(kStepBlock) void k_step(StepArgs<typename S::R> a,
+                                                     const typename S::T* __restrict__ src,
+                                                     typename S::T* __restrict__ dst,
+                                                     double* __restrict__ fpart) {
+  using R = typename S::R;
+  const size_t N = a.N;
+  const uint32_t bid = blockIdx.x;
+
+  if (MODE != LBM_MODE_COLLIDE && int(bid) < a.nlb) {
+    // ---------------- link cells: Bouzidi + force partials ----------------
+    const int b = int(bid) * kStepBlock + int(threadIdx.x);
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    if (b < a.n_links) {
+      const uint32_t c = uint32_t(a.link_cell[b]);
+      int x, y, z;
+      cell_coords(a, c, x, y, z);
+      R own[Q], s[Q];
+#pragma unroll
+      for (int i = 0; i < Q; ++i) own[i] = S::ld_c(src, i * N + c, a.off[i]);
+      s[0] = own[0];
+#pragma unroll
+      for (int i = 1; i < Q; ++i) {
+        const int io = opp(i);
+        const R qo = R(a.link_q[size_t(io - 1) * a.n_links + b]);
+        if (qo >= R(0)) {
+          if (qo >= R(0.5)) {
+            const R inv2q = R(1) / (R(2) * fmin(fmax(qo, R(0.5)), R(1)));
+            s[i] = inv2q * own[io] + (R(1) - inv2q) * own[i];
+          } else {
+            const R twoq = R(2) * fmin(fmax(qo, R(0)), R(0.5));
+            const R ffv = S::ld_c(src, io * N + wrap_next(a, i, x, y, z), a.off[io]);
+            s[i] = twoq * own[io] + (R(1) - twoq) * ffv;
+          }
+        } else {
+          s[i] = S::ld_c(src, i * N + pull_src(a, i, c, x, y, z), a.off[i]);
+        }
+      }
+      boundaries(a, s, x);
+      // momentum exchange: links j with q_j(x) >= 0 (forces.py:95-118)
+#pragma unroll
+      for (int j = 1; j < Q; ++j) {
+        if (a.link_q[size_t(j - 1) * a.n_links + b] >= 0.0) {
+          const double t = double(own[j]) + double(s[opp(j)]);
+          if (EX[j]) fx += EX[j] > 0 ? t : -t;
+          if (EY[j]) fy += EY[j] > 0 ? t : -t;
+          if (EZ[j]) fz += EZ[j] > 0 ? t : -t;
+        }
+      }
+      if (MODE == LBM_MODE_FULL) collide_cell<R, COLL>(a, s, c);
+#pragma unroll
+      for (int i = 0; i < Q; ++i) S::st(dst, i * N + c, s[i], a.off[i]);
+    }
+    if (fpart) {
+      __shared__ double red[3][kStepBlock / 64];
+      fx = wave_sum(fx);
+      fy = wave_sum(fy);
+      fz = wave_sum(fz);
+      const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+      if (lane == 0) {
+        red[0][w] = fx;
+        red[1][w] = fy;
+        red[2][w] = fz;
+      }
+      __syncthreads();
+      if (threadIdx.x < 3) {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < kStepBlock / 64; ++k) acc += red[threadIdx.x][k];
+        fpart[size_t(bid) * 3 + threadIdx.x] = acc;
+      }
+    }
+    return;
+  }
+
+  // ---------------- dense sweep ----------------
+  const uint32_t first = (MODE == LBM_MODE_COLLIDE) ? 0u : uint32_t(a.nlb);
+  const uint32_t tile = xcd_swizzle(bid - first, uint32_t(a.ndb));
+  const uint32_t c = tile * kStepBlock + threadIdx.x;
+  if (c >= a.N) return;
+  R s[Q];
+  if (MODE == LBM_MODE_COLLIDE) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) s[i] = S::ld(src, i * N + c, a.off[i]);
+    collide_cell<R, COLL>(a, s, c);
+  } else {
+    if (a.flags && (a.flags[c] & 2)) return;  // owned by a link workgroup
+    int x, y, z;
+    cell_coords(a, c, x, y, z);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) s[i] = S::ld_c(src, i * N + pull_src(a, i, c, x, y, z), a.off[i]);
+    boundaries(a, s, x);
+    if (MODE == LBM_MODE_FULL) collide_cell<R, COLL>(a, s, c);
+  }
+#pragma unroll
+  for (int i = 0; i < Q; ++i) S::st(dst, i * N + c, s[i], a.off[i]);
+}
+
+__global__ void k_force_reduce(const double* __restrict__ part, int T, int slots, double* __restrict__ force) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T) return;
+  double fx = 0, fy = 0, fz = 0;
+  const double* p = part + size_t(t) * slots * 3;
+  for (int s = 0; s < slots; ++s) {
+    fx += p[3 * s];
+    fy += p[3 * s + 1];
+    fz += p[3 * s + 2];
+  }
+  force[3 * t] = fx;
+  force[3 * t + 1] = fy;
+  force[3 * t + 2] = fz;
+}
+
+template <class SI, class SO>
+__global__ void k_convert(const typename SI::T* __restrict__ in, typename SO::T* __restrict__ out, StepArgs<float> a,
+                          size_t total) {
+  const size_t k = blockIdx.x * size_t(blockDim.x) + threadIdx.x;
+  if (k >= total) return;
+  const int i = int(k / a.N);
+  using RI = typename SI::R;
+  using RO = typename SO::R;
+  const RI v = SI::ld_c(in, k, RI(a.off[i]));
+  SO::st(out, k, RO(v), RO(a.off[i]));
+}
+
+template <typename R>
+int make_args(const lbm_step_desc* d, int mode, StepArgs<R>& a) {
+  if (!d || d->nx <= 0 || d->ny <= 0 || d->nz <= 0) return LBM_E_ARG;
+  const long long n = 1LL * d->nx * d->ny * d->nz;
+  if (n * Q >= (1LL << 31)) return LBM_E_ARG;
+  a.nx = d->nx;
+  a.ny = d->ny;
+  a.nz = d->nz;
+  a.N = uint32_t(n);
+  a.dnx = make_fastdiv(uint32_t(d->nx));
+  a.dny = make_fastdiv(uint32_t(d->ny));
+  a.coll = d->collision;
+  a.smag = d->smagorinsky;
+  a.bounce = d->bounce;
+  a.guard = d->guard;
+  a.periodic = d->periodic_yz;
+  a.n_links = d->n_links;
+  a.tau = R(d->tau);
+  a.cs = R(d->smag_cs);
+  for (int k = 0; k < 3; ++k) {
+    a.uin[k] = R(d->u_in[k]);
+    a.uref[k] = R(d->u_ref[k]);
+  }
+  a.rho_out = R(d->rho_out);
+  for (int i = 0; i < Q; ++i) a.off[i] = R(d->offset[i]);
+  a.flags = d->flags;
+  a.phi = static_cast<const R*>(d->porosity);
+  a.link_cell = d->link_cell;
+  a.link_q = d->link_q;
+  if (a.bounce == LBM_BOUNCE_HARD && !a.flags) return LBM_E_ARG;
+  if (a.bounce == LBM_BOUNCE_SOFT && !a.phi) return LBM_E_ARG;
+  if (a.n_links < 0 || (a.n_links > 0 && (!a.link_cell || !a.link_q || !a.flags))) return LBM_E_ARG;
+  if (a.coll != LBM_COLL_BGK && a.coll != LBM_COLL_MRT) return LBM_E_ARG;
+  if (a.guard < 0 || a.guard > 2 || a.bounce < 0 || a.bounce > 2) return LBM_E_ARG;
+  a.nlb = (mode == LBM_MODE_COLLIDE) ? 0 : (a.n_links + kStepBlock - 1) / kStepBlock;
+  a.ndb = int((n + kStepBlock - 1) / kStepBlock);
+  return LBM_OK;
+}
+
+template <class S>
+int launch_step(const lbm_step_desc* d, int mode, const void* src, void* dst, double* fpart, hipStream_t s) {
+  using R = typename S::R;
+  StepArgs<R> a;
+  const int rc = make_args(d, mode, a);
+  if (rc) return rc;
+  if (!src || !dst || src == dst) return LBM_E_ARG;
+  const unsigned grid = unsigned(a.nlb + a.ndb);
+  const auto* in = static_cast<const typename S::T*>(src);
+  auto* out = static_cast<typename S::T*>(dst);
+#define LBM_STEP_LAUNCH(COLL, MODE) \
+  hipLaunchKernelGGL((k_step<S, COLL, MODE>), grid, kStepBlock, 0, s, a, in, out, fpart)
+  if (a.coll == LBM_COLL_MRT) {
+    if (mode == LBM_MODE_FULL) LBM_STEP_LAUNCH(LBM_COLL_MRT, LBM_MODE_FULL);
+    else if (mode == LBM_MODE_COLLIDE) LBM_STEP_LAUNCH(LBM_COLL_MRT, LBM_MODE_COLLIDE);
+    else if (mode == LBM_MODE_STREAM) LBM_STEP_LAUNCH(LBM_COLL_MRT, LBM_MODE_STREAM);
+    else return LBM_E_ARG;
+  } else {
+    if (mode == LBM_MODE_FULL) LBM_STEP_LAUNCH(LBM_COLL_BGK, LBM_MODE_FULL);
+    else if (mode == LBM_MODE_COLLIDE) LBM_STEP_LAUNCH(LBM_COLL_BGK, LBM_MODE_COLLIDE);
+    else if (mode == LBM_MODE_STREAM) LBM_STEP_LAUNCH(LBM_COLL_BGK, LBM_MODE_STREAM);
+    else return LBM_E_ARG;
+  }
+#undef LBM_STEP_LAUNCH
+  return hipGetLastError() == hipSuccess ? LBM_OK : LBM_E_LAUNCH;
+}
+
+}  // namespace lbm
+
+using namespace lbm;
+
+extern "C" {
+
+int lbm_step_force_slots(const lbm_step_desc* d) {
+  if (!d || d->n_links < 0) return 0;
+  return (d->n_links + kStepBlock - 1) / kStepBlock;
+}
+
+int lbm_step(const lbm_step_desc* d, int mode, const void* src, void* dst, double* force_partial,
+             lbm_stream_t stream) {
+  if (!d) return LBM_E_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  switch (d->storage) {
+    case LBM_F32: return launch_step<StoreF32>(d, mode, src, dst, force_partial, s);
+    case LBM_F64: return launch_step<StoreF64>(d, mode, src, dst, force_partial, s);
+    case LBM_F16DEV: return launch_step<StoreF16>(d, mode, src, dst, force_partial, s);
+    case LBM_BF16DEV: return launch_step<StoreBF16>(d, mode, src, dst, force_partial, s);
+  }
+  return LBM_E_DTYPE;
+}
+
+int lbm_force_reduce(const double* partials, int T, int slots, double* force, lbm_stream_t stream) {
+  if (T <= 0 || !force || slots < 0 || (slots > 0 && !partials)) return LBM_E_ARG;
+  hipLaunchKernelGGL(k_force_reduce, (T + 255) / 256, 256, 0, (hipStream_t)stream, partials, T, slots, force);
+  return hipGetLastError() == hipSuccess ? LBM_OK : LBM_E_LAUNCH;
+}
+
+int lbm_convert_state(const lbm_step_desc* d, int src_dtype, const void* src, int dst_dtype, void* dst,
+                      lbm_stream_t stream) {
+  StepArgs<float> a;
+  const int rc = make_args(d, LBM_MODE_COLLIDE, a);
+  if (rc) return rc;
+  if (!src || !dst) return LBM_E_ARG;
+  const size_t total = size_t(a.N) * Q;
+  const unsigned grid = unsigned((total + 255) / 256);
+  hipStream_t s = (hipStream_t)stream;
+#define LBM_CONV(SI, SO) hipLaunchKernelGGL((k_convert<SI, SO>), grid, 256, 0, s, \
+                                            (const SI::T*)src, (SO::T*)dst, a, total)
+  if (src_dtype == LBM_F32 && dst_dtype == LBM_F16DEV) LBM_CONV(StoreF32, StoreF16);
+  else if (src_dtype == LBM_F32 && dst_dtype == LBM_BF16DEV) LBM_CONV(StoreF32, StoreBF16);
+  else if (src_dtype == LBM_F16DEV && dst_dtype == LBM_F32) LBM_CONV(StoreF16, StoreF32);
+  else if (src_dtype == LBM_BF16DEV && dst_dtype == LBM_F32) LBM_CONV(StoreBF16, StoreF32);
+  else if (src_dtype == LBM_F32 && dst_dtype == LBM_F64) LBM_CONV(StoreF32, StoreF64);
+  else if (src_dtype == LBM_F64 && dst_dtype == LBM_F32) LBM_CONV(StoreF64, StoreF32);
+  else return LBM_E_DTYPE;
+#undef LBM_CONV
+  return hipGetLastError() == hipSuccess ? LBM_OK : LBM_E_LAUNCH;
+}
+
+int lbm_device_check(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return LBM_E_ARG;
+  hipFuncAttributes attr;
+  const hipError_t e = hipFuncGetAttributes(&attr, reinterpret_cast<const void*>(&k_force_reduce));
+  return e == hipSuccess ? LBM_OK : int(e);
+}
+
+}  // extern "C"

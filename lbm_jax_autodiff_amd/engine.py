@@ -1,0 +1,264 @@
+"""Fused rollout engine: the hot path of jax_lbm on liblbm_hip.so.
+
+A ``Lattice`` compiles one geometry and configuration (collision model,
+bounce mode, guard variant, storage precision) into the ``lbm_step_desc`` of
+the C ABI and runs timesteps with one kernel launch each.  The carried state
+is the post-collision population P_t; ``run`` turns an initial state S_0 into
+(S_N, P_{N-1}, forces[N, 3]) exactly as the reference's scan loops return
+(f_final, f_post) and compute momentum_exchange(f_post, f_final, q) every step
+(examples/10_cd_convergence.py:69-97, examples/11_optimisation_v2.py:141-167,
+jax_lbm/state.py:87-230).
+"""
+
+from __future__ import annotations
+
+import ctypes
+from typing import Optional, Sequence
+
+import torch
+
+from . import _field as F
+from . import _lib as L
+
+_STORAGE = {
+    "f32": (L.LBM_F32, torch.float32, torch.float32),
+    "f64": (L.LBM_F64, torch.float64, torch.float64),
+    "f16": (L.LBM_F16DEV, torch.float16, torch.float32),
+    "bf16": (L.LBM_BF16DEV, torch.bfloat16, torch.float32),
+}
+_GUARD = {"none": L.LBM_GUARD_NONE, "post": L.LBM_GUARD_POST, "pre": L.LBM_GUARD_PRE}
+_W = [1.0 / 3.0] + [1.0 / 18.0] * 6 + [1.0 / 36.0] * 12
+_E = [(0, 0, 0), (1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1), (1, 1, 0), (-1, 1, 0),
+      (1, -1, 0), (-1, -1, 0), (1, 0, 1), (-1, 0, 1), (1, 0, -1), (-1, 0, -1), (0, 1, 1), (0, -1, 1), (0, 1, -1),
+      (0, -1, -1)]
+
+
+def _f32(x: float) -> float:
+    return float(torch.tensor(x, dtype=torch.float32).item())
+
+
+def equilibrium_values(rho: float, u: Sequence[float]):
+    """The 19 values of f_eq(rho, u) in float64 (host; used for offsets)."""
+    ux, uy, uz = (float(v) for v in u)
+    usq = ux * ux + uy * uy + uz * uz
+    out = []
+    for w, (ex, ey, ez) in zip(_W, _E):
+        eu = ex * ux + ey * uy + ez * uz
+        out.append(_f32(w) * rho * (1.0 + 3.0 * eu + 4.5 * eu * eu - 1.5 * usq))
+    return out
+
+
+class Links:
+    """Compiled Bouzidi links of a dense q: link cells, their 18 q values and flags."""
+
+    def __init__(self, n, cell, q, dq, flags, dense_q):
+        self.n = n
+        self.cell = cell          # (n,) int32 cell index (lattice storage)
+        self.q = q                # (18, n) float64
+        self.dq = dq              # (18, n) float64 or None
+        self.flags = flags        # (N,) uint8
+        self.dense_q = dense_q    # lattice-storage q used to build them
+
+
+def compile_links(q_storage: Optional[torch.Tensor], solid_storage: Optional[torch.Tensor], nx, ny, nz, device,
+                  dqdr_storage: Optional[torch.Tensor] = None) -> Links:
+    """Dense q (lattice storage, float32/float64) -> link list + per-cell flags."""
+    N = nx * ny * nz
+    if q_storage is None:
+        flags = torch.zeros(N, dtype=torch.uint8, device=device)
+        if solid_storage is not None:
+            flags = (solid_storage.reshape(-1) >= 1).to(torch.uint8)
+        return Links(0, None, None, None, flags, None)
+    tag = F.dtype_tag(q_storage.dtype)
+    nb = (N + 255) // 256
+    counts = torch.empty(nb, dtype=torch.int32, device=device)
+    offsets = torch.empty(nb, dtype=torch.int32, device=device)
+    total = torch.empty(1, dtype=torch.int32, device=device)
+    stream = L.stream_handle(device)
+    L.call("lbm_links_count", tag, L.ptr(q_storage), L.ptr(counts), L.ptr(offsets), L.ptr(total), nx, ny, nz, stream)
+    n = int(total.item())
+    cell = torch.empty(max(n, 1), dtype=torch.int32, device=device)
+    lq = torch.empty((18, max(n, 1)), dtype=torch.float64, device=device)
+    ldq = torch.empty((18, max(n, 1)), dtype=torch.float64, device=device) if dqdr_storage is not None else None
+    flags = torch.empty(N, dtype=torch.uint8, device=device)
+    if dqdr_storage is not None:
+        dqdr_storage = dqdr_storage.to(q_storage.dtype).contiguous()
+    L.call("lbm_links_build", tag, L.ptr(q_storage), L.ptr(dqdr_storage), L.ptr(solid_storage), L.ptr(offsets), n,
+           L.ptr(cell), L.ptr(lq), L.ptr(ldq), L.ptr(flags), nx, ny, nz, stream)
+    if n == 0:
+        cell, lq, ldq = None, None, None
+    return Links(n, cell, lq, ldq, flags, q_storage)
+
+
+class Lattice:
+    """One compiled configuration of the fused D3Q19 step.
+
+    Parameters mirror LBMParams (jax_lbm/state.py:57-84) plus the variants the
+    reference's drivers use: ``bounce`` hard (solid >= 1 reversal,
+    state.py:146-149) or soft (porosity blend, smooth.py:151-155);
+    ``guard`` none, post (state.py:158-169, reset to eq(1, 0)) or pre
+    (examples/10:71-75, reset to eq(1, guard_velocity)).
+    ``storage`` f32 / f64 / f16 / bf16 (half formats store f_i - offset_i
+    with float32 arithmetic); ``offset`` "weights" (w_i) or "inlet"
+    (f_eq(1, inlet_velocity)) or 19 numbers.
+    """
+
+    def __init__(self, nx, ny, nz, *, tau, inlet_velocity=(0.05, 0.0, 0.0), rho_out=1.0, collision="bgk",
+                 smagorinsky_cs=None, solid=None, porosity=None, q=None, guard="none", guard_velocity=None,
+                 periodic_yz=True, storage="f32", offset="weights", device=None, dqdr=None):
+        if storage not in _STORAGE:
+            raise ValueError(f"storage must be one of {sorted(_STORAGE)}")
+        if guard not in _GUARD:
+            raise ValueError(f"guard must be one of {sorted(_GUARD)}")
+        if collision not in ("bgk", "mrt"):
+            raise ValueError("collision must be 'bgk' or 'mrt'")
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        self.N = self.nx * self.ny * self.nz
+        self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        self.storage = storage
+        self.tag, self.buf_dtype, self.real = _STORAGE[storage]
+        self.collision = collision
+        self.guard = guard
+        self.inlet_velocity = [float(v) for v in inlet_velocity]
+        self.tau = F.scalar_float(tau)
+
+        solid_s = F.scalar_storage(solid.to(torch.int32)) if solid is not None else None
+        self.solid_storage = solid_s
+        self.porosity_storage = None
+        if porosity is not None:
+            self.porosity_storage = F.scalar_storage(porosity.detach().to(self.real)).to(self.device)
+        qs = None
+        if q is not None:
+            qs = F.pop_storage(q.detach()).to(self.device)
+            if qs.dtype not in (torch.float32, torch.float64):
+                qs = qs.to(torch.float32)
+        dq_s = F.pop_storage(dqdr.detach()).to(self.device) if dqdr is not None else None
+        self.links = compile_links(qs, solid_s, self.nx, self.ny, self.nz, self.device, dq_s)
+
+        d = L.StepDesc()
+        d.nx, d.ny, d.nz = self.nx, self.ny, self.nz
+        d.storage = self.tag
+        d.collision = L.LBM_COLL_MRT if collision == "mrt" else L.LBM_COLL_BGK
+        d.smagorinsky = 1 if smagorinsky_cs is not None else 0
+        d.smag_cs = float(smagorinsky_cs) if smagorinsky_cs is not None else 0.1
+        if porosity is not None:
+            d.bounce = L.LBM_BOUNCE_SOFT
+        elif solid is not None:
+            d.bounce = L.LBM_BOUNCE_HARD
+        else:
+            d.bounce = L.LBM_BOUNCE_NONE
+        d.guard = _GUARD[guard]
+        d.periodic_yz = 1 if periodic_yz else 0
+        d.n_links = self.links.n
+        d.tau = self.tau
+        for k in range(3):
+            d.u_in[k] = self.inlet_velocity[k]
+        d.rho_out = float(rho_out)
+        gv = self.inlet_velocity if guard_velocity is None else [float(v) for v in guard_velocity]
+        for k in range(3):
+            d.u_ref[k] = gv[k]
+        if isinstance(offset, str):
+            if offset == "weights":
+                offs = [_f32(w) for w in _W]
+            elif offset == "inlet":
+                offs = equilibrium_values(1.0, self.inlet_velocity)
+            else:
+                raise ValueError("offset must be 'weights', 'inlet' or 19 numbers")
+        else:
+            offs = [float(v) for v in offset]
+        if storage in ("f16", "bf16"):
+            offs = [_f32(v) for v in offs]
+        else:
+            offs = [0.0] * 19
+        for i in range(19):
+            d.offset[i] = offs[i]
+        d.flags = L.ptr(self.links.flags)
+        d.porosity = L.ptr(self.porosity_storage)
+        d.link_cell = L.ptr(self.links.cell)
+        d.link_q = L.ptr(self.links.q)
+        self.desc = d
+        self.slots = int(L.load().lbm_step_force_slots(ctypes.byref(d)))
+
+    # ---- state buffers ---------------------------------------------------
+    def new_state(self) -> torch.Tensor:
+        return torch.empty((19, self.nz, self.ny, self.nx), dtype=self.buf_dtype, device=self.device)
+
+    def to_state(self, f: torch.Tensor) -> torch.Tensor:
+        """(19, nx, ny, nz) float tensor -> state buffer in the storage format."""
+        F.require_cuda(f, "f")
+        fs = F.pop_storage(f)
+        if self.storage in ("f32", "f64"):
+            return fs.to(self.buf_dtype).clone() if fs.dtype != self.buf_dtype else fs.clone()
+        src = fs.to(torch.float32).contiguous()
+        out = self.new_state()
+        L.call("lbm_convert_state", ctypes.byref(self.desc), L.LBM_F32, L.ptr(src), self.tag, L.ptr(out),
+               L.stream_handle(self.device))
+        return out
+
+    def from_state(self, buf: torch.Tensor) -> torch.Tensor:
+        """State buffer -> float32/float64 tensor indexed (19, nx, ny, nz)."""
+        if self.storage in ("f32", "f64"):
+            return F.pop_view(buf)
+        out = torch.empty((19, self.nz, self.ny, self.nx), dtype=torch.float32, device=self.device)
+        L.call("lbm_convert_state", ctypes.byref(self.desc), self.tag, L.ptr(buf), L.LBM_F32, L.ptr(out),
+               L.stream_handle(self.device))
+        return F.pop_view(out)
+
+    # ---- single launches -------------------------------------------------
+    def launch(self, mode: int, src: torch.Tensor, dst: torch.Tensor, fpart: Optional[torch.Tensor] = None):
+        L.call("lbm_step", ctypes.byref(self.desc), int(mode), L.ptr(src), L.ptr(dst), L.ptr(fpart),
+               L.stream_handle(self.device))
+
+    def collide(self, s_buf, p_buf):
+        """P_t = bounce(collide(S_t)) (rollout prologue)."""
+        self.launch(L.LBM_MODE_COLLIDE, s_buf, p_buf)
+
+    def step(self, p_in, p_out, fpart=None):
+        """P_{t+1} from P_t in one launch; fpart receives this step's force partials."""
+        self.launch(L.LBM_MODE_FULL, p_in, p_out, fpart)
+
+    def stream(self, p_in, s_out, fpart=None):
+        """S_{t+1} from P_t (rollout epilogue)."""
+        self.launch(L.LBM_MODE_STREAM, p_in, s_out, fpart)
+
+    def reduce_forces(self, fpart: torch.Tensor, T: int) -> torch.Tensor:
+        force = torch.empty((T, 3), dtype=torch.float64, device=self.device)
+        if T > 0:
+            L.call("lbm_force_reduce", L.ptr(fpart), int(T), int(self.slots), L.ptr(force),
+                   L.stream_handle(self.device))
+        return force
+
+    # ---- rollouts --------------------------------------------------------
+    def run(self, f0: torch.Tensor, n_steps: int, *, forces: bool = True, keep_every: int = 0,
+            keep: Optional[list] = None):
+        """Run n_steps timesteps from S_0 = f0.
+
+        Returns (S_N, P_{N-1}, F) with F[t] = momentum_exchange(P_t, S_{t+1})
+        in float64 (None if forces=False).  When ``keep`` is a list, the P_t
+        buffers with t % keep_every == 0 are appended to it (checkpoints).
+        """
+        n_steps = int(n_steps)
+        if n_steps < 1:
+            raise ValueError("n_steps must be >= 1")
+        s0 = self.to_state(f0)
+        a = self.new_state()
+        b = self.new_state()
+        fpart = None
+        if forces and self.slots > 0:
+            fpart = torch.zeros((n_steps, self.slots, 3), dtype=torch.float64, device=self.device)
+        self.collide(s0, a)
+        cur, nxt = a, b
+        for t in range(n_steps - 1):
+            if keep is not None and keep_every and t % keep_every == 0:
+                keep.append((t, cur.clone()))
+            self.step(cur, nxt, None if fpart is None else fpart[t])
+            cur, nxt = nxt, cur
+        if keep is not None and keep_every and (n_steps - 1) % keep_every == 0:
+            keep.append((n_steps - 1, cur.clone()))
+        s_final = s0  # reuse the prologue input buffer for S_N
+        self.stream(cur, s_final, None if fpart is None else fpart[n_steps - 1])
+        F_hist = None
+        if forces:
+            F_hist = self.reduce_forces(fpart, n_steps) if fpart is not None else torch.zeros(
+                (n_steps, 3), dtype=torch.float64, device=self.device)
+        return self.from_state(s_final), self.from_state(cur), F_hist

@@ -1,0 +1,129 @@
+"""Fused-step parity: Lattice.run (one kernel launch per step) against the
+oracle's unfused rollout (oracle.hot_step), for every variant of the step."""
+
+import itertools
+
+import pytest
+import torch
+
+from lbm_jax_autodiff_amd.engine import Lattice
+from oracle import lbm_oracle as O
+from _helpers import near_eq_state, random_q, cpu, maxabs
+
+pytestmark = pytest.mark.gpu
+
+NX, NY, NZ = 14, 9, 6
+
+
+def _oracle(f0, steps, **kw):
+    with torch.no_grad():
+        return O.rollout(f0, steps, forces=True, **kw)
+
+
+def _run(dev, f0, steps, storage, collision, bounce, guard, smag, periodic, q, solid, phi, tau=0.6,
+         vel=(0.05, 0.0, 0.0), uref=None):
+    lat = Lattice(NX, NY, NZ, tau=tau, inlet_velocity=vel, collision=collision,
+                  smagorinsky_cs=0.1 if smag else None,
+                  solid=solid.to(dev) if (bounce == "hard") else None,
+                  porosity=phi.to(dev) if bounce == "soft" else None,
+                  q=q.to(dev) if q is not None else None, guard=guard, guard_velocity=uref,
+                  periodic_yz=periodic, storage=storage, device=dev)
+    dt = torch.float64 if storage == "f64" else torch.float32
+    return lat.run(f0.to(dt).to(dev), steps)
+
+
+CASES = list(itertools.product(["bgk", "mrt"], ["none", "hard", "soft"], ["none", "post", "pre"], [False, True],
+                               [True, False], [0.1, 0.0]))
+
+
+@pytest.mark.parametrize("collision,bounce,guard,smag,periodic,p_wall", CASES)
+def test_fused_matches_oracle_f64(dev, collision, bounce, guard, smag, periodic, p_wall):
+    steps = 6
+    f0 = near_eq_state(NX, NY, NZ, amp=2e-3)
+    q = random_q(NX, NY, NZ, seed=4, p_wall=p_wall)
+    g = torch.Generator().manual_seed(2)
+    solid = (torch.rand((NX, NY, NZ), generator=g) < 0.1).to(torch.int32)
+    phi = 0.3 * torch.rand((NX, NY, NZ), generator=g, dtype=torch.float64)
+    vel = torch.tensor([0.05, 0.0, 0.0], dtype=torch.float64)
+    uref = (0.05, 0.0, 0.0)
+    feq_ref = O.equilibrium(torch.ones((NX, NY, NZ), dtype=torch.float64),
+                            torch.stack([torch.full((NX, NY, NZ), 0.05, dtype=torch.float64),
+                                         torch.zeros((NX, NY, NZ), dtype=torch.float64),
+                                         torch.zeros((NX, NY, NZ), dtype=torch.float64)]))
+    s_ref, p_ref, f_ref = _oracle(
+        f0, steps, tau=torch.tensor(0.6, dtype=torch.float64), velocity=vel, q=q,
+        solid=solid if bounce == "hard" else None, porosity=phi if bounce == "soft" else None,
+        collision=collision, smagorinsky_cs=0.1 if smag else None, periodic_yz=periodic, guard=guard,
+        f_eq_ref=feq_ref)
+    s, p, fh = _run(dev, f0, steps, "f64", collision, bounce, guard, smag, periodic, q, solid, phi, uref=uref)
+    assert maxabs(s, s_ref) < 1e-12
+    assert maxabs(p, p_ref) < 1e-12
+    assert maxabs(fh, f_ref) < 1e-11
+
+
+@pytest.mark.parametrize("collision", ["bgk", "mrt"])
+def test_fused_f32_close_to_f64_oracle(dev, collision):
+    steps = 20
+    f0 = near_eq_state(NX, NY, NZ, amp=2e-3)
+    q = random_q(NX, NY, NZ, seed=4, p_wall=0.1)
+    vel = torch.tensor([0.05, 0.0, 0.0], dtype=torch.float64)
+    s_ref, p_ref, f_ref = _oracle(f0, steps, tau=torch.tensor(0.6, dtype=torch.float64), velocity=vel, q=q,
+                                  collision=collision)
+    s, p, fh = _run(dev, f0, steps, "f32", collision, "none", "none", False, True, q, None, None)
+    assert maxabs(s, s_ref) < 5e-6
+    assert maxabs(p, p_ref) < 5e-6
+    assert maxabs(fh, f_ref) < 1e-4
+
+
+@pytest.mark.parametrize("storage,tol", [("f16", 3e-4), ("bf16", 3e-3)])
+def test_half_storage_close_to_f32(dev, storage, tol):
+    steps = 10
+    f0 = near_eq_state(NX, NY, NZ, amp=2e-3)
+    q = random_q(NX, NY, NZ, seed=4, p_wall=0.1)
+    s32, p32, f32 = _run(dev, f0, steps, "f32", "mrt", "none", "none", False, True, q, None, None)
+    sh, ph, fh = _run(dev, f0, steps, storage, "mrt", "none", "none", False, True, q, None, None)
+    assert maxabs(sh, s32) < tol
+    assert maxabs(ph, p32) < tol
+
+
+def test_guard_resets_only_bad_cells(dev):
+    """Forced NaN / out-of-range cells are reset, the others untouched (state.py:158-169)."""
+    f0 = near_eq_state(NX, NY, NZ, amp=1e-3)
+    bad = [(3, 2, 1), (7, 4, 3), (10, 8, 5)]
+    f0[:, 3, 2, 1] = float("nan")
+    f0[:, 7, 4, 3] *= 3.0   # rho ~ 3 > 2
+    f0[1, 10, 8, 5] += 0.5  # |u| > 0.4
+    vel = torch.tensor([0.05, 0.0, 0.0], dtype=torch.float64)
+    s_ref, p_ref, _ = _oracle(f0, 1, tau=torch.tensor(0.6, dtype=torch.float64), velocity=vel,
+                              q=-torch.ones((19, NX, NY, NZ), dtype=torch.float64), collision="bgk", guard="pre",
+                              f_eq_ref=O.equilibrium(torch.ones((NX, NY, NZ), dtype=torch.float64),
+                                                     torch.zeros((3, NX, NY, NZ), dtype=torch.float64)))
+    lat = Lattice(NX, NY, NZ, tau=0.6, collision="bgk", guard="pre", guard_velocity=(0, 0, 0), storage="f64",
+                  device=dev)
+    s, p, _ = lat.run(f0.to(dev), 1)
+    p = cpu(p)
+    assert torch.isfinite(p).all()
+    assert maxabs(p, p_ref) < 1e-12
+    w = O.weights(torch.float64)
+    for (x, y, z) in bad:
+        # reset to eq(1, 0) = w before collision, which BGK leaves unchanged
+        assert float((p[:, x, y, z] - w).abs().max()) < 1e-15
+
+
+def test_poiseuille_kat(dev):
+    """60x20x4, tau=0.8, 3000 BGK steps: the reference's published profile to 6
+    decimals (README.md:59-69, docs/getting_started/first_simulation.md:101-109)."""
+    nx, ny, nz = 60, 20, 4
+    solid = O.channel_walls(nx, ny, nz)
+    q = O.compute_q_from_solid(solid, nx, ny, nz, 0.5)
+    f0 = O.equilibrium(torch.ones((nx, ny, nz), dtype=torch.float32), torch.zeros((3, nx, ny, nz)))
+    lat = Lattice(nx, ny, nz, tau=0.8, collision="bgk", solid=solid.to(dev), q=q.to(dev), storage="f32",
+                  device=dev)
+    s, _, _ = lat.run(f0.to(dev), 3000, forces=False)
+    s = cpu(s).to(torch.float64)
+    rho = s.sum(0)
+    ux = (s * O.velocities(torch.float64)[:, 0].reshape(19, 1, 1, 1)).sum(0) / rho
+    prof = ux[nx // 2].mean(-1)
+    published = {0: 0.000200, 1: 0.008247, 5: 0.057873, 9: 0.076879, 10: 0.076879, 15: 0.048352, 19: 0.000200}
+    for y, v in published.items():
+        assert abs(float(prof[y]) - v) <= 6e-7, (y, float(prof[y]), v)

@@ -1,0 +1,174 @@
+"""Per-operator parity: HIP kernels (through the C ABI) against the CPU oracle.
+
+Float64 kernels must match the float64 oracle to ~1e-13 (same formulas,
+different summation order); float32 kernels must match it to float32
+rounding (~1e-6 on O(1) populations).
+"""
+
+import pytest
+import torch
+
+import lbm_jax_autodiff_amd as lbm
+from oracle import lbm_oracle as O
+from _helpers import near_eq_state, random_q, cpu, maxabs
+
+pytestmark = pytest.mark.gpu
+
+NX, NY, NZ = 10, 7, 5
+TOL = {torch.float64: 1e-12, torch.float32: 2e-6}
+
+
+@pytest.fixture(params=[torch.float64, torch.float32], ids=["f64", "f32"])
+def dt(request):
+    return request.param
+
+
+def test_equilibrium_density_velocity(dev, dt):
+    f = near_eq_state(NX, NY, NZ)
+    rho = O.compute_density(f)
+    u = O.compute_velocity(f, rho)
+    feq_ref = O.equilibrium(rho, u)
+    feq = lbm.equilibrium(rho.to(dt).to(dev), u.to(dt).to(dev))
+    assert feq.shape == (19, NX, NY, NZ)
+    assert maxabs(feq, feq_ref) < TOL[dt]
+    fd = f.to(dt).to(dev)
+    assert maxabs(lbm.compute_density(fd), rho) < 4 * TOL[dt]
+    assert maxabs(lbm.compute_velocity(fd, lbm.compute_density(fd)), u) < 4 * TOL[dt]
+
+
+@pytest.mark.parametrize("op", ["bgk", "mrt"])
+@pytest.mark.parametrize("field_tau", [False, True])
+def test_collision(dev, dt, op, field_tau):
+    f = near_eq_state(NX, NY, NZ, amp=3e-3)
+    if field_tau:
+        g = torch.Generator().manual_seed(5)
+        tau = 0.55 + 0.3 * torch.rand((NX, NY, NZ), generator=g, dtype=torch.float64)
+    else:
+        tau = torch.tensor(0.6, dtype=torch.float64)
+    ref = getattr(O, op)(f, tau)
+    tau_d = tau.to(dt).to(dev) if field_tau else 0.6
+    out = getattr(lbm, op)(f.to(dt).to(dev), tau_d)
+    assert maxabs(out, ref) < 4 * TOL[dt]
+
+
+def test_smagorinsky(dev, dt):
+    f = near_eq_state(NX, NY, NZ, amp=5e-3)
+    ref = O.smagorinsky_tau(f, 0.55, 0.1)
+    out = lbm.smagorinsky_tau(f.to(dt).to(dev), 0.55, 0.1)
+    assert maxabs(out, ref) < 10 * TOL[dt]
+    assert bool((cpu(out) >= 0.55 - 1e-6).all())
+
+
+def test_bounce_ops(dev, dt):
+    f = near_eq_state(NX, NY, NZ)
+    g = torch.Generator().manual_seed(3)
+    phi = torch.rand((NX, NY, NZ), generator=g, dtype=torch.float64)
+    assert maxabs(lbm.soft_bounce_back(f.to(dt).to(dev), phi.to(dt).to(dev)), O.soft_bounce_back(f, phi)) < TOL[dt]
+    solid = (torch.rand((NX, NY, NZ), generator=g) < 0.3).to(torch.int32)
+    solid[0, 0, 0] = 2
+    ref = torch.where((solid >= 1).unsqueeze(0), f[list(O.OPPOSITE)], f)
+    out = lbm.core.hard_bounce_back(f.to(dt).to(dev), solid.to(dev))
+    assert maxabs(out, ref) == 0.0 or maxabs(out, ref) < TOL[dt]
+
+
+@pytest.mark.parametrize("periodic", [True, False])
+def test_stream_bouzidi(dev, dt, periodic):
+    f = near_eq_state(NX, NY, NZ)
+    q = random_q(NX, NY, NZ)
+    ref = O.stream(f, q, periodic)
+    out = lbm.stream(f.to(dt).to(dev), q.to(dt).to(dev), periodic)
+    assert maxabs(out, ref) < 2 * TOL[dt]
+
+
+def test_stream_halfway_identity(dev, dt):
+    """q = 1/2 at a link: both Bouzidi branches give f*_iopp (streaming.py:120-139)."""
+    f = near_eq_state(NX, NY, NZ)
+    q = -torch.ones((19, NX, NY, NZ), dtype=torch.float64)
+    q[2, 3] = 0.5  # wall along -x seen from x=3: direction 1 bounces
+    out = cpu(lbm.stream(f.to(dt).to(dev), q.to(dt).to(dev)))
+    assert maxabs(out[1, 3], f[2, 3]) < TOL[dt]
+
+
+def test_zou_he(dev, dt):
+    f = near_eq_state(NX, NY, NZ)
+    vel = torch.tensor([0.05, 0.01, -0.005], dtype=torch.float64)
+    assert maxabs(lbm.zou_he_inlet(f.to(dt).to(dev), vel), O.zou_he_inlet(f, vel)) < 4 * TOL[dt]
+    assert maxabs(lbm.zou_he_inlet(f.to(dt).to(dev), vel, rho_target=1.02),
+                  O.zou_he_inlet(f, vel, rho_target=1.02)) < 4 * TOL[dt]
+    assert maxabs(lbm.zou_he_outlet(f.to(dt).to(dev), 1.0), O.zou_he_outlet(f, 1.0)) < 4 * TOL[dt]
+    # closed-form post-conditions (algebra of boundary.py:113-122, 192-198)
+    g = cpu(lbm.zou_he_inlet(f.to(dt).to(dev), vel)).to(torch.float64)
+    rho_in = g[:, 0].sum(0)
+    jx = (g[:, 0] * O.velocities(torch.float64)[:, 0].reshape(19, 1, 1)).sum(0)
+    assert float((jx - rho_in * 0.05).abs().max()) < 10 * TOL[dt]
+    h = cpu(lbm.zou_he_outlet(f.to(dt).to(dev), 1.0)).to(torch.float64)
+    assert float((h[:, -1].sum(0) - 1.0).abs().max()) < 10 * TOL[dt]
+
+
+def test_momentum_exchange(dev, dt):
+    fp = near_eq_state(NX, NY, NZ, seed=7)
+    fs = near_eq_state(NX, NY, NZ, seed=8)
+    q = random_q(NX, NY, NZ, seed=9)
+    ref = O.momentum_exchange(fp, fs, q)
+    out = lbm.momentum_exchange(fp.to(dt).to(dev), fs.to(dt).to(dev), q.to(dt).to(dev))
+    assert maxabs(out, ref) < 100 * TOL[dt]
+    tot, pre = lbm.momentum_exchange_decomposed(fp.to(dt).to(dev), fs.to(dt).to(dev), q.to(dt).to(dev))
+    rt, rp = O.momentum_exchange_decomposed(fp, fs, q)
+    assert maxabs(tot, rt) < 100 * TOL[dt] and maxabs(pre, rp) < 100 * TOL[dt]
+    none = -torch.ones_like(q)
+    assert maxabs(lbm.momentum_exchange(fp.to(dt).to(dev), fs.to(dt).to(dev), none.to(dt).to(dev)),
+                  torch.zeros(3)) == 0.0
+
+
+def test_projected_area_and_cd(dev):
+    solid = torch.zeros((16, 16, 16), dtype=torch.int32)
+    solid[6:10, 6:10, 6:10] = 1  # box 4x4 in the y-z shadow (test_forces.py:62-67)
+    assert float(lbm.projected_area(solid.to(dev), 0)) == 16.0
+    ground = torch.zeros((8, 8, 8), dtype=torch.int32)
+    ground[:, :, 0] = 2
+    assert float(lbm.projected_area(ground.to(dev), 0)) == 0.0
+    for ax in (1, 2):
+        assert float(lbm.projected_area(solid.to(dev), ax)) == float(O.projected_area(solid, ax))
+    force = torch.tensor([0.01, 0.0, 0.0], dtype=torch.float64, device=dev)
+    cd = lbm.drag_coefficient(force, 0.05, 100.0)
+    assert abs(float(cd) - 0.08) < 1e-9  # test_forces.py:86-91
+
+
+@pytest.mark.parametrize("center", [(-2.0, 0.0, 0.0), (0.0, 0.0, 0.0), (-1.9375, 0.0625, 0.0625)])
+def test_create_sphere(dev, center):
+    nx, ny, nz = 32, 16, 16
+    solid_ref, q_ref = O.create_sphere(nx, ny, nz, center, 0.5)
+    solid, q = lbm.create_sphere(nx, ny, nz, center, 0.5, device=dev)
+    assert torch.equal(cpu(solid), solid_ref)
+    assert maxabs(q, q_ref) < 2e-6
+
+
+def test_smooth_sphere_and_q_from_solid(dev):
+    nx, ny, nz = 32, 16, 16
+    por_ref, solid_ref, q_ref = O.smooth_sphere_geometry(nx, ny, nz, (0.0, 0.0, 0.0), 0.5, 20.0)
+    por, solid, q = lbm.smooth_sphere_geometry(nx, ny, nz, (0.0, 0.0, 0.0), 0.5, 20.0, device=dev)
+    assert torch.equal(cpu(solid), solid_ref)
+    assert maxabs(por, por_ref) < 1e-6
+    assert maxabs(q, q_ref) < 2e-6
+    qh = lbm.compute_q_from_solid(solid, nx, ny, nz, 0.5)
+    assert torch.equal(cpu(qh), O.compute_q_from_solid(solid_ref, nx, ny, nz, 0.5))
+    sdf = lbm.signed_distance_sphere(nx, ny, nz, (0.0, 0.0, 0.0), 0.5, device=dev)
+    assert maxabs(sdf, O.signed_distance_sphere(nx, ny, nz, (0.0, 0.0, 0.0), 0.5)) < 1e-6
+    assert maxabs(lbm.porosity_from_sdf(sdf, 20.0), por_ref) < 1e-6
+
+
+def test_smooth_sphere_radius_gradient(dev):
+    """dL/dr through porosity(r) and q(r) against torch autograd of the FP64 oracle."""
+    nx, ny, nz = 32, 16, 16
+    g = torch.Generator().manual_seed(11)
+    wq = torch.randn((19, nx, ny, nz), generator=g, dtype=torch.float64)
+    wp = torch.randn((nx, ny, nz), generator=g, dtype=torch.float64)
+    r = torch.tensor(0.5, dtype=torch.float64, requires_grad=True)
+    por, _, q = O.smooth_sphere_geometry(nx, ny, nz, (0.0, 0.0, 0.0), r, 20.0, coord_dtype=torch.float64)
+    (gref,) = torch.autograd.grad((por * wp).sum() + (torch.where(q >= 0, q, torch.zeros_like(q)) * wq).sum(), r)
+    rd = torch.tensor(0.5, dtype=torch.float64, device=dev, requires_grad=True)
+    por_d, _, q_d = lbm.smooth_sphere_geometry(nx, ny, nz, (0.0, 0.0, 0.0), rd, 20.0, coord_dtype=torch.float64,
+                                               out_dtype=torch.float64)
+    loss = (por_d * wp.to(dev)).sum() + (torch.where(q_d >= 0, q_d, torch.zeros_like(q_d)) * wq.to(dev)).sum()
+    (gd,) = torch.autograd.grad(loss, rd)
+    assert abs(float(gd) - float(gref)) < 1e-9 * max(1.0, abs(float(gref)))
