@@ -246,6 +246,40 @@ int lbm_step(const lbm_step_desc* d, int mode, const void* src, void* dst,
 int lbm_force_reduce(const double* partials, int T, int slots, double* force,
                      lbm_stream_t stream);
 
+/* ---- Adjoint of the fused timestep --------------------------------------
+ * The reference differentiates its rollout with jax.grad / value_and_grad
+ * (examples/11_optimisation_v2.py:176, tests/test_gradients_f64.py:129-217);
+ * these entry points are the hand-written transpose of lbm_step that
+ * replaces that XLA transpose.  Cotangent buffers ("lambda") are float for
+ * the float32/half storages and double for LBM_F64, in lattice storage. */
+
+typedef struct lbm_adjoint_desc {
+  double* dq_acc;          /* 18 x n_links: += dL/dq of each link (may be null) */
+  void* mu_link;           /* 19 x n_links scratch: per-step link-cell cotangents */
+  double* phi_acc;         /* N: += dL/dporosity (soft bounce; may be null) */
+  double* u_acc;           /* 3 x (ny*nz): += dL/du_in per inlet cell (may be null) */
+  int32_t n_targets;       /* size of the scatter plan */
+  const int32_t* tgt_index;/* n_targets: lambda entry i*N + cell */
+  const int32_t* tgt_start;/* n_targets + 1: CSR offsets into src/coef */
+  const int32_t* src;      /* >= 0: mu_link index i*n_links + b; < 0: force term of direction -1-src */
+  const double* coef;      /* multiplier of mu_link[src] */
+} lbm_adjoint_desc;
+
+/* One adjoint step (one launch).  mode LBM_MODE_FULL: state = P_t,
+ * lam_in = dL/dP_{t+1} -> lam_out = owner part of dL/dP_t.  mode
+ * LBM_MODE_STREAM: lam_in = dL/dS_{t+1} (the epilogue).  mode
+ * LBM_MODE_COLLIDE: state = S_0, lam_in = dL/dP_0 -> lam_out = dL/dS_0.
+ * gF (3 doubles on the device, may be null) = dL/dF_t of this step's
+ * momentum-exchange force.  Requires periodic_yz and nx >= 2.  Must be
+ * followed by lbm_adjoint_scatter for modes FULL and STREAM. */
+int lbm_step_adjoint(const lbm_step_desc* d, const lbm_adjoint_desc* a, int mode,
+                     const void* state, const void* lam_in, void* lam_out,
+                     const double* gF, lbm_stream_t stream);
+
+/* Adds the Bouzidi-link and force-term cotangents to lam (same step's gF). */
+int lbm_adjoint_scatter(const lbm_step_desc* d, const lbm_adjoint_desc* a, void* lam,
+                        const double* gF, lbm_stream_t stream);
+
 /* Convert between storage formats (e.g. float32 <-> half deviations) for a
  * full 19N state. */
 int lbm_convert_state(const lbm_step_desc* d, int src_dtype, const void* src,

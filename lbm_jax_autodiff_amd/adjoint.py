@@ -1,0 +1,322 @@
+"""Reverse mode over a whole rollout with the hand-written HIP adjoint.
+
+``rollout(f0, n_steps, q=..., porosity=..., inlet_velocity=..., ...)`` runs the
+fused forward (engine.Lattice) and returns (S_N, P_{N-1}, F[N, 3]) as torch
+tensors that are differentiable with respect to ``f0``, the dense Bouzidi
+``q``, the soft-bounce ``porosity`` and the inlet velocity.  It replaces
+``jax.grad`` / ``jax.value_and_grad`` over the reference's scan loops
+(examples/11_optimisation_v2.py:141-176, tests/test_gradients_f64.py:95-217)
+and its sqrt(N) ``jax.checkpoint`` chunking (examples/08_gradient_convergence.py:49-66).
+
+Memory policies (``checkpoint``):
+  "all"      keep every P_t (store-all);
+  K (int)    keep P_t every K steps, recompute each K-step segment once;
+  (K1, K2)   two levels: keep every K1, recompute a segment keeping every
+             K2, recompute each K2 sub-segment keeping all.
+Recomputation is bit-identical to the forward (same kernels, no atomics).
+"""
+
+from __future__ import annotations
+
+import ctypes
+from typing import Optional, Sequence, Union
+
+import numpy as np
+import torch
+
+from . import _field as F
+from . import _lib as L
+from .engine import Lattice
+
+_E = np.array([[0, 0, 0], [1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1], [1, 1, 0], [-1, 1, 0],
+               [1, -1, 0], [-1, -1, 0], [1, 0, 1], [-1, 0, 1], [1, 0, -1], [-1, 0, -1], [0, 1, 1], [0, -1, 1],
+               [0, 1, -1], [0, -1, -1]], dtype=np.int64)
+_OPP = np.array([0, 2, 1, 4, 3, 6, 5, 10, 9, 8, 7, 14, 13, 12, 11, 18, 17, 16, 15], dtype=np.int64)
+
+
+class AdjointDesc(ctypes.Structure):
+    """Mirror of ``lbm_adjoint_desc`` (include/lbm_hip.h)."""
+
+    _fields_ = [
+        ("dq_acc", ctypes.c_void_p), ("mu_link", ctypes.c_void_p), ("phi_acc", ctypes.c_void_p),
+        ("u_acc", ctypes.c_void_p), ("n_targets", ctypes.c_int32), ("tgt_index", ctypes.c_void_p),
+        ("tgt_start", ctypes.c_void_p), ("src", ctypes.c_void_p), ("coef", ctypes.c_void_p),
+    ]
+
+
+def scatter_plan(lat: Lattice):
+    """Per-entry CSR of the cotangent contributions outside the owner writes.
+
+    For each link (cell x, wall direction j, arriving direction i = opp(j)) of
+    the reference's Bouzidi rule (streaming.py:113-181):
+      q >= 1/2: S_i(x) = P_j(x)/(2q) + (1 - 1/(2q)) P_i(x)
+      q <  1/2: S_i(x) = 2q P_j(x) + (1 - 2q) P_j(x + e_i)   (x + e_i wrapped)
+    and the force term F += e_j P_j(x) (forces.py:95-118).  Entries are
+    ordered by target and then by source, so the sums are deterministic.
+    """
+    n = lat.links.n
+    if n == 0:
+        return None
+    nx, ny, nz, N = lat.nx, lat.ny, lat.nz, lat.N
+    cell = lat.links.cell.detach().to("cpu").numpy().astype(np.int64)
+    q = lat.links.q.detach().to("cpu").numpy()  # (18, n)
+    x = cell % nx
+    y = (cell // nx) % ny
+    z = cell // (nx * ny)
+    keys, srcs, coefs = [], [], []
+    b = np.arange(n, dtype=np.int64)
+    for j in range(1, 19):
+        qj = q[j - 1]
+        has = qj >= 0.0
+        if not has.any():
+            continue
+        i = _OPP[j]
+        bb, qq, cc = b[has], qj[has], cell[has]
+        src = i * n + bb
+        hi = qq >= 0.5
+        inv2q = 1.0 / (2.0 * np.clip(qq, 0.5, 1.0))
+        twoq = 2.0 * np.clip(qq, 0.0, 0.5)
+        # own P_j(x): both branches
+        keys.append(j * N + cc)
+        srcs.append(src)
+        coefs.append(np.where(hi, inv2q, twoq))
+        # own P_i(x): high branch
+        keys.append(i * N + cc[hi])
+        srcs.append(src[hi])
+        coefs.append(1.0 - inv2q[hi])
+        # P_j(x + e_i): low branch
+        lo = ~hi
+        fx = (x[has][lo] + _E[i, 0]) % nx
+        fy = (y[has][lo] + _E[i, 1]) % ny
+        fz = (z[has][lo] + _E[i, 2]) % nz
+        keys.append(j * N + (fz * ny + fy) * nx + fx)
+        srcs.append(src[lo])
+        coefs.append(1.0 - twoq[lo])
+        # force term on P_j(x)
+        keys.append(j * N + cc)
+        srcs.append(np.full(cc.shape, -1 - j, dtype=np.int64))
+        coefs.append(np.zeros(cc.shape))
+    key = np.concatenate(keys)
+    src = np.concatenate(srcs)
+    coef = np.concatenate(coefs)
+    order = np.lexsort((src, key))
+    key, src, coef = key[order], src[order], coef[order]
+    uniq, start = np.unique(key, return_index=True)
+    start = np.concatenate([start, [key.size]]).astype(np.int32)
+    dev = lat.device
+    return (torch.from_numpy(uniq.astype(np.int32)).to(dev), torch.from_numpy(start).to(dev),
+            torch.from_numpy(src.astype(np.int32)).to(dev), torch.from_numpy(coef).to(dev))
+
+
+class _Checkpoints:
+    """Yields P_t for t = N-1 down to 0 under a memory policy."""
+
+    def __init__(self, lat: Lattice, policy, n_steps: int):
+        self.lat = lat
+        self.N = n_steps
+        if policy == "all" or policy is None:
+            self.levels = [1]
+        elif isinstance(policy, int):
+            self.levels = [max(1, int(policy)), 1]
+        else:
+            k1, k2 = (int(v) for v in policy)
+            self.levels = [max(1, k1), max(1, min(k2, k1)), 1]
+        self.top = {}
+
+    @property
+    def stride(self):
+        return self.levels[0]
+
+    def record(self, t, buf):
+        """Called by the forward for every t; keeps what the top level needs."""
+        if t % self.stride == 0:
+            self.top[t] = buf.clone()
+
+    def _recompute(self, p_start, t0, t1, keep_every):
+        """{t: P_t} for t = t0 and every t0 < t < t1 with t % keep_every == 0, from P_t0."""
+        lat = self.lat
+        out = {t0: p_start}
+        cur = p_start
+        sa = sb = None
+        if keep_every > 1:
+            sa, sb = lat.new_state(), lat.new_state()
+        for t in range(t0, t1 - 1):
+            if keep_every == 1:
+                dst = lat.new_state()
+                lat.step(cur, dst)
+                out[t + 1] = dst
+                cur = dst
+            else:
+                dst = sa if cur is not sa else sb
+                lat.step(cur, dst)
+                if (t + 1) % keep_every == 0:
+                    out[t + 1] = dst.clone()
+                cur = dst
+        return out
+
+    def descending(self):
+        """Generator of (t, P_t) for t from N-1 down to 0."""
+        yield from self._desc(self.top, 0, self.N, 0)
+
+    def _desc(self, have, t0, t1, level):
+        if self.levels[level] == 1:
+            for t in range(t1 - 1, t0 - 1, -1):
+                yield t, have[t]
+            return
+        starts = sorted(t for t in have if t0 <= t < t1)
+        ends = starts[1:] + [t1]
+        for s, e in reversed(list(zip(starts, ends))):
+            sub = self._recompute(have[s], s, e, self.levels[level + 1])
+            yield from self._desc(sub, s, e, level + 1)
+            del sub
+
+
+class _RolloutFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, f0, q, porosity, u_in, cfg):
+        n_steps = cfg["n_steps"]
+        vel = [float(v) for v in u_in.detach().reshape(-1).to("cpu", torch.float64).tolist()]
+        lat = Lattice(cfg["nx"], cfg["ny"], cfg["nz"], tau=cfg["tau"], inlet_velocity=vel, rho_out=cfg["rho_out"],
+                      collision=cfg["collision"], smagorinsky_cs=cfg["smagorinsky_cs"], solid=cfg["solid"],
+                      porosity=porosity.detach() if porosity is not None else None,
+                      q=q.detach() if q is not None else None, guard=cfg["guard"],
+                      guard_velocity=cfg["guard_velocity"], periodic_yz=True, storage=cfg["storage"],
+                      offset=cfg["offset"], device=f0.device)
+        ck = _Checkpoints(lat, cfg["checkpoint"], n_steps)
+        s0 = lat.to_state(f0.detach())
+        a, b = lat.new_state(), lat.new_state()
+        fpart = torch.zeros((n_steps, max(lat.slots, 1), 3), dtype=torch.float64, device=lat.device)
+        lat.collide(s0, a)
+        cur, nxt = a, b
+        for t in range(n_steps - 1):
+            ck.record(t, cur)
+            lat.step(cur, nxt, fpart[t] if lat.slots else None)
+            cur, nxt = nxt, cur
+        ck.record(n_steps - 1, cur)
+        s_final = lat.new_state()
+        lat.stream(cur, s_final, fpart[n_steps - 1] if lat.slots else None)
+        forces = lat.reduce_forces(fpart, n_steps) if lat.slots else torch.zeros(
+            (n_steps, 3), dtype=torch.float64, device=lat.device)
+        p_last = cur.clone() if lat.storage in ("f16", "bf16") else cur
+        ctx.lat, ctx.ck, ctx.s0 = lat, ck, s0
+        ctx.cfg = cfg
+        ctx.has_q = q is not None
+        ctx.has_phi = porosity is not None
+        ctx.f0_dtype = f0.dtype
+        ctx.q_dtype = q.dtype if q is not None else None
+        ctx.phi_dtype = porosity.dtype if porosity is not None else None
+        ctx.u_dtype = u_in.dtype
+        s_out = lat.from_state(s_final)
+        p_out = lat.from_state(p_last)
+        if lat.storage in ("f32", "f64"):
+            s_out = s_out.clone()
+            p_out = p_out.clone()
+        return s_out.to(f0.dtype), p_out.to(f0.dtype), forces
+
+    @staticmethod
+    def backward(ctx, gS, gP, gF):
+        lat, ck = ctx.lat, ctx.ck
+        n_steps = ctx.cfg["n_steps"]
+        real = torch.float64 if lat.storage == "f64" else torch.float32
+        dev = lat.device
+        nx, ny, nz, N = lat.nx, lat.ny, lat.nz, lat.N
+        n = lat.links.n
+        want_q = ctx.needs_input_grad[1] and ctx.has_q and n > 0
+        want_phi = ctx.needs_input_grad[2] and ctx.has_phi
+        want_u = ctx.needs_input_grad[3]
+        want_f0 = ctx.needs_input_grad[0]
+
+        ad = AdjointDesc()
+        keep = []
+        mu = torch.zeros((19, max(n, 1)), dtype=real, device=dev)
+        keep.append(mu)
+        ad.mu_link = L.ptr(mu)
+        dq = None
+        if want_q:
+            dq = torch.zeros((18, n), dtype=torch.float64, device=dev)
+            ad.dq_acc = L.ptr(dq)
+        phi_acc = None
+        if want_phi:
+            phi_acc = torch.zeros(N, dtype=torch.float64, device=dev)
+            ad.phi_acc = L.ptr(phi_acc)
+        u_acc = None
+        if want_u:
+            u_acc = torch.zeros((ny * nz, 3), dtype=torch.float64, device=dev)
+            ad.u_acc = L.ptr(u_acc)
+        plan = scatter_plan(lat)
+        if plan is not None:
+            keep.extend(plan)
+            ad.n_targets = int(plan[0].numel())
+            ad.tgt_index, ad.tgt_start, ad.src, ad.coef = (L.ptr(t) for t in plan)
+        if gF is None:
+            gF = torch.zeros((n_steps, 3), dtype=torch.float64, device=dev)
+        gF = gF.detach().to(torch.float64).contiguous()
+        stream = L.stream_handle(dev)
+        dptr = ctypes.byref(lat.desc)
+        aptr = ctypes.byref(ad)
+
+        lam_a = torch.empty((19, nz, ny, nx), dtype=real, device=dev)
+        lam_b = torch.empty_like(lam_a)
+        g_in = F.pop_storage(gS.detach()).to(real) if gS is not None else torch.zeros_like(lam_a)
+        g_in = g_in.contiguous()
+        it = ck.descending()
+        t, p_t = next(it)
+        assert t == n_steps - 1
+        L.call("lbm_step_adjoint", dptr, aptr, L.LBM_MODE_STREAM, L.ptr(p_t), L.ptr(g_in), L.ptr(lam_a),
+               L.ptr(gF[t]), stream)
+        L.call("lbm_adjoint_scatter", dptr, aptr, L.ptr(lam_a), L.ptr(gF[t]), stream)
+        if gP is not None:
+            lam_a += F.pop_storage(gP.detach()).to(real)
+        cur, nxt = lam_a, lam_b
+        for t, p_t in it:
+            L.call("lbm_step_adjoint", dptr, aptr, L.LBM_MODE_FULL, L.ptr(p_t), L.ptr(cur), L.ptr(nxt),
+                   L.ptr(gF[t]), stream)
+            L.call("lbm_adjoint_scatter", dptr, aptr, L.ptr(nxt), L.ptr(gF[t]), stream)
+            cur, nxt = nxt, cur
+        # cur = dL/dP_0 -> prologue transpose
+        g_f0 = None
+        if want_f0 or want_phi:
+            L.call("lbm_step_adjoint", dptr, aptr, L.LBM_MODE_COLLIDE, L.ptr(ctx.s0), L.ptr(cur), L.ptr(nxt),
+                   None, stream)
+            if want_f0:
+                g_f0 = F.pop_view(nxt).to(ctx.f0_dtype).clone()
+        g_q = None
+        if want_q:
+            dense = torch.zeros((19, N), dtype=torch.float64, device=dev)
+            cells = lat.links.cell.long()
+            dense[1:, :].index_copy_(1, cells, dq)
+            g_q = F.pop_view(dense.reshape(19, nz, ny, nx)).to(ctx.q_dtype)
+        g_phi = F.scalar_view(phi_acc.reshape(nz, ny, nx)).to(ctx.phi_dtype) if want_phi else None
+        g_u = u_acc.sum(0).to(ctx.u_dtype) if want_u else None
+        return g_f0, g_q, g_phi, g_u, None
+
+
+def rollout(f0: torch.Tensor, n_steps: int, *, tau, q: Optional[torch.Tensor] = None,
+            porosity: Optional[torch.Tensor] = None, solid: Optional[torch.Tensor] = None,
+            inlet_velocity: Union[Sequence[float], torch.Tensor] = (0.05, 0.0, 0.0), rho_out: float = 1.0,
+            collision: str = "bgk", smagorinsky_cs: Optional[float] = None, guard: str = "none",
+            guard_velocity=None, storage: str = "f32", offset="weights",
+            checkpoint: Union[str, int, Sequence[int]] = "all"):
+    """Differentiable fused rollout: (S_N, P_{N-1}, forces[N, 3] float64).
+
+    Step order as in the reference's drivers: [pre guard] -> [Smagorinsky]
+    -> BGK|MRT -> soft bounce (porosity) or hard bounce (solid >= 1) -> stream
+    with Bouzidi (q) -> Zou-He inlet/outlet -> [post guard].
+    """
+    nx, ny, nz = F.dims_of_pop(f0)
+    if isinstance(inlet_velocity, torch.Tensor):
+        u = inlet_velocity.reshape(3)
+    else:
+        u = torch.tensor([float(v) for v in inlet_velocity], dtype=torch.float64, device=f0.device)
+    cfg = dict(n_steps=int(n_steps), nx=nx, ny=ny, nz=nz, tau=F.scalar_float(tau), rho_out=float(rho_out),
+               collision=collision, smagorinsky_cs=smagorinsky_cs, solid=solid, guard=guard,
+               guard_velocity=guard_velocity, storage=storage, offset=offset, checkpoint=checkpoint)
+    if int(n_steps) < 1:
+        raise ValueError("n_steps must be >= 1")
+    return _RolloutFn.apply(f0, q, porosity, u, cfg)
+
+
+def drag_series(forces: torch.Tensor, inlet_velocity: float, area, rho: float = 1.0) -> torch.Tensor:
+    """Cd_t = |F_t,x| / (0.5 rho U^2 A + 1e-10) per step, differentiable in the
+    forces (jax_lbm/core/forces.py:237-276 applied to each row)."""
+    return forces[:, 0].abs() / (0.5 * rho * inlet_velocity * inlet_velocity * area + 1e-10)

@@ -1,0 +1,481 @@
+// Hand-written adjoint of the fused D3Q19 step (reverse mode over one step).
+//
+// Given P_t and lambda_{t+1} = dL/dP_{t+1}, one launch recomputes each cell's
+// forward step (pull, Bouzidi, Zou-He, guards, collision, bounce) and pulls
+// the cotangent back to mu = dL/d(pulled values).  Pull streaming is a
+// bijection once the x faces and the y/z axes wrap: entry (y, i) of P_t is
+// consumed normally by exactly one cell x = y + e_i.  That "owner" writes
+// lambda_t(y)_i with a plain store: mu_i for a normal pull, 0 where the
+// reference keeps the cell's own value at the x faces (Zou-He overwrites
+// those populations, so they carry no cotangent) or where a Bouzidi link
+// replaces the pull.  Every entry of lambda_t is written exactly once, with
+// no atomics.  The extra consumers -- Bouzidi links (P(x)_iopp, P(x)_i,
+// P(x+e_i)_iopp) and the momentum-exchange force (P(x)_j) -- are added
+// afterwards by lbm_adjoint_scatter from a fixed per-entry plan, so the
+// result is deterministic.
+#include "lbm_adjoint_physics.hpp"
+
+namespace lbm {
+
+constexpr int kAdjBlock = 256;
+
+template <typename R>
+struct AdjArgs {
+  // forward configuration (same meaning as StepArgs)
+  int nx, ny, nz;
+  uint32_t N;
+  FastDiv dnx, dny;
+  int smag, bounce, guard, n_links, nlb, ndb;
+  R tau, cs;
+  R uin[3];
+  R rho_out;
+  R uref[3];
+  R off[Q];
+  const uint8_t* flags;
+  const R* phi;
+  const int32_t* link_cell;
+  const double* link_q;
+  // adjoint outputs / scratch
+  double* dq_acc;
+  R* mu_link;
+  double* phi_acc;
+  double* u_acc;
+};
+
+template <typename R>
+__device__ __forceinline__ void adj_coords(const AdjArgs<R>& a, uint32_t c, int& x, int& y, int& z) {
+  const uint32_t yz = fdiv(c, a.dnx);
+  x = int(c - yz * uint32_t(a.nx));
+  const uint32_t zz = fdiv(yz, a.dny);
+  y = int(yz - zz * uint32_t(a.ny));
+  z = int(zz);
+}
+
+// cell x - e_i with all three axes wrapped: the entry whose normal consumer is x
+template <typename R>
+__device__ __forceinline__ uint32_t owner_target(const AdjArgs<R>& a, int i, int x, int y, int z) {
+  int sx = x - EX[i], sy = y - EY[i], sz = z - EZ[i];
+  sx = sx < 0 ? sx + a.nx : (sx >= a.nx ? sx - a.nx : sx);
+  sy = sy < 0 ? sy + a.ny : (sy >= a.ny ? sy - a.ny : sy);
+  sz = sz < 0 ? sz + a.nz : (sz >= a.nz ? sz - a.nz : sz);
+  return (uint32_t(sz) * uint32_t(a.ny) + uint32_t(sy)) * uint32_t(a.nx) + uint32_t(sx);
+}
+
+template <typename R>
+__device__ __forceinline__ uint32_t next_cell(const AdjArgs<R>& a, int i, int x, int y, int z) {
+  int fx = x + EX[i], fy = y + EY[i], fz = z + EZ[i];
+  fx = fx < 0 ? fx + a.nx : (fx >= a.nx ? fx - a.nx : fx);
+  fy = fy < 0 ? fy + a.ny : (fy >= a.ny ? fy - a.ny : fy);
+  fz = fz < 0 ? fz + a.nz : (fz >= a.nz ? fz - a.nz : fz);
+  return (uint32_t(fz) * uint32_t(a.ny) + uint32_t(fy)) * uint32_t(a.nx) + uint32_t(fx);
+}
+
+__device__ __forceinline__ bool x_fallback(int i, int x, int nx) {
+  return (EX[i] == 1 && x == 0) || (EX[i] == -1 && x == nx - 1);
+}
+
+// Pulled values of cell x (normal pulls and, for link cells, Bouzidi).
+template <class S, bool LINK>
+__device__ __forceinline__ void pull_cell(const AdjArgs<typename S::R>& a, const typename S::T* __restrict__ P,
+                                          uint32_t c, int x, int y, int z, int b, typename S::R* s,
+                                          typename S::R* own) {
+  using R = typename S::R;
+  const size_t N = a.N;
+  if (LINK) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) own[i] = S::ld_c(P, i * N + c, a.off[i]);
+    s[0] = own[0];
+  } else {
+    s[0] = S::ld_c(P, c, a.off[0]);
+  }
+#pragma unroll
+  for (int i = 1; i < Q; ++i) {
+    bool link = false;
+    R qo = R(-1);
+    if (LINK) {
+      qo = R(a.link_q[size_t(opp(i) - 1) * a.n_links + b]);
+      link = qo >= R(0);
+    }
+    if (link) {
+      const int io = opp(i);
+      if (qo >= R(0.5)) {
+        const R inv2q = R(1) / (R(2) * fmin(fmax(qo, R(0.5)), R(1)));
+        s[i] = inv2q * own[io] + (R(1) - inv2q) * own[i];
+      } else {
+        const R twoq = R(2) * fmin(fmax(qo, R(0)), R(0.5));
+        s[i] = twoq * own[io] + (R(1) - twoq) * S::ld_c(P, io * N + next_cell(a, i, x, y, z), a.off[io]);
+      }
+    } else {
+      const uint32_t src = x_fallback(i, x, a.nx) ? c : owner_target(a, i, x, y, z);
+      s[i] = S::ld_c(P, i * N + src, a.off[i]);
+    }
+  }
+}
+
+template <typename R>
+__device__ __forceinline__ void apply_zou_he(const AdjArgs<R>& a, R* s, int x) {
+  if (x == 0) zou_he_in(s, a.uin[0], a.uin[1], a.uin[2], false, R(0));
+  if (x == a.nx - 1) zou_he_out(s, a.rho_out);
+}
+
+// One cell: forward recompute + cotangent pull-back + owner writes.
+// MODE FULL: lam_in = dL/dP_{t+1}; MODE STREAM: lam_in = dL/dS_{t+1}.
+template <class S, int COLL, int MODE, bool LINK>
+__device__ __forceinline__ void adjoint_cell(const AdjArgs<typename S::R>& a, const typename S::T* __restrict__ P,
+                                             const typename S::R* __restrict__ lam_in,
+                                             typename S::R* __restrict__ lam_out, const double* __restrict__ gF,
+                                             uint32_t c, int b) {
+  using R = typename S::R;
+  const size_t N = a.N;
+  int x, y, z;
+  adj_coords(a, c, x, y, z);
+  R s[Q], own[LINK ? Q : 1];
+  pull_cell<S, LINK>(a, P, c, x, y, z, b, s, own);
+  apply_zou_he(a, s, x);
+  const bool post_reset = (a.guard == LBM_GUARD_POST) && guard_bad(s);
+  if (post_reset) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) s[i] = R(weight(i));
+  }
+  R g[Q];
+  if (MODE == LBM_MODE_FULL) {
+    bool pre_reset = false;
+    if (a.guard == LBM_GUARD_PRE && guard_bad(s)) {
+      pre_reset = true;
+      const R uu = a.uref[0] * a.uref[0] + a.uref[1] * a.uref[1] + a.uref[2] * a.uref[2];
+#pragma unroll
+      for (int i = 0; i < Q; ++i) s[i] = feq(i, R(1), a.uref[0], a.uref[1], a.uref[2], uu);
+    }
+#pragma unroll
+    for (int i = 0; i < Q; ++i) g[i] = lam_in[i * N + c];
+    // bounce^T
+    if (a.bounce == LBM_BOUNCE_HARD && (a.flags[c] & 1)) {
+#pragma unroll
+      for (int i = 1; i < Q; ++i) {
+        if (opp(i) > i) {
+          const R t = g[i];
+          g[i] = g[opp(i)];
+          g[opp(i)] = t;
+        }
+      }
+    } else if (a.bounce == LBM_BOUNCE_SOFT) {
+      const R p = a.phi[c];
+      R cc[Q];
+      const R tau_e = a.smag ? smag_tau(s, a.tau, a.cs) : a.tau;
+#pragma unroll
+      for (int i = 0; i < Q; ++i) cc[i] = s[i];
+      if (COLL == LBM_COLL_MRT) collide_mrt(cc, R(1) / tau_e);
+      else collide_bgk(cc, R(1) / tau_e);
+      R gphi = R(0);
+#pragma unroll
+      for (int i = 0; i < Q; ++i) gphi += g[i] * (cc[opp(i)] - cc[i]);
+      if (a.phi_acc) a.phi_acc[c] += double(gphi);
+      R t[Q];
+#pragma unroll
+      for (int i = 0; i < Q; ++i) t[i] = g[i];
+#pragma unroll
+      for (int i = 0; i < Q; ++i) g[i] = (R(1) - p) * t[i] + p * t[opp(i)];
+    }
+    // collision^T (and the Smagorinsky tau_eff path)
+    if (pre_reset) {
+#pragma unroll
+      for (int i = 0; i < Q; ++i) g[i] = R(0);
+    } else {
+      const R tau_e = a.smag ? smag_tau(s, a.tau, a.cs) : a.tau;
+      const R omega = R(1) / tau_e;
+      const R gom = (COLL == LBM_COLL_MRT) ? mrt_vjp(s, g, omega) : bgk_vjp(s, g, omega);
+      if (a.smag) smag_vjp(s, g, a.tau, a.cs, -gom * omega * omega);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) g[i] = lam_in[i * N + c];
+  }
+  // momentum-exchange force uses S_{t+1}(x)_opp(j) for links j (forces.py:95-118)
+  if (LINK && gF) {
+    const R gx = R(gF[0]), gy = R(gF[1]), gz = R(gF[2]);
+#pragma unroll
+    for (int j = 1; j < Q; ++j) {
+      if (a.link_q[size_t(j - 1) * a.n_links + b] >= 0.0) {
+        R v = R(0);
+        if (EX[j]) v += EX[j] > 0 ? gx : -gx;
+        if (EY[j]) v += EY[j] > 0 ? gy : -gy;
+        if (EZ[j]) v += EZ[j] > 0 ? gz : -gz;
+        g[opp(j)] += v;
+      }
+    }
+  }
+  if (post_reset) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) g[i] = R(0);
+  }
+  // Zou-He^T (outlet was applied last)
+  if (x == 0 || x == a.nx - 1) {
+    R sp[Q], tmp[LINK ? Q : 1];
+    pull_cell<S, LINK>(a, P, c, x, y, z, b, sp, tmp);
+    if (x == a.nx - 1) zou_he_out_vjp(g, a.rho_out);
+    if (x == 0) {
+      double gu[3] = {0.0, 0.0, 0.0};
+      zou_he_in_vjp(sp, g, a.uin[0], a.uin[1], a.uin[2], gu);
+      if (a.u_acc) {
+        const size_t row = size_t(c) / size_t(a.nx);
+        a.u_acc[3 * row] += gu[0];
+        a.u_acc[3 * row + 1] += gu[1];
+        a.u_acc[3 * row + 2] += gu[2];
+      }
+    }
+  }
+  // owner writes and Bouzidi bookkeeping
+  lam_out[c] = g[0];
+#pragma unroll
+  for (int i = 1; i < Q; ++i) {
+    bool link = false;
+    R qo = R(-1);
+    if (LINK) {
+      qo = R(a.link_q[size_t(opp(i) - 1) * a.n_links + b]);
+      link = qo >= R(0);
+    }
+    const uint32_t tgt = owner_target(a, i, x, y, z);
+    const bool normal = !link && !x_fallback(i, x, a.nx);
+    lam_out[i * N + tgt] = normal ? g[i] : R(0);
+    if (LINK) {
+      a.mu_link[size_t(i) * a.n_links + b] = link ? g[i] : R(0);
+      if (link && a.dq_acc) {
+        // d S_i / d q for the taken branch; 0 outside clip bounds
+        const int io = opp(i);
+        R dsdq = R(0);
+        if (qo >= R(0.5)) {
+          if (qo <= R(1)) dsdq = -(own[io] - own[i]) / (R(2) * qo * qo);
+        } else {
+          const R cval = S::ld_c(P, io * N + next_cell(a, i, x, y, z), a.off[io]);
+          dsdq = R(2) * (own[io] - cval);
+        }
+        a.dq_acc[size_t(io - 1) * a.n_links + b] += double(g[i]) * double(dsdq);
+      }
+    }
+  }
+}
+
+template <class S, int COLL, int MODE>
+__global__ __launch_bounds__(kAdjBlock) void k_adjoint(AdjArgs<typename S::R> a, const typename S::T* __restrict__ P,
+                                                       const typename S::R* __restrict__ lam_in,
+                                                       typename S::R* __restrict__ lam_out,
+                                                       const double* __restrict__ gF) {
+  const uint32_t bid = blockIdx.x;
+  if (int(bid) < a.nlb) {
+    const int b = int(bid) * kAdjBlock + int(threadIdx.x);
+    if (b >= a.n_links) return;
+    adjoint_cell<S, COLL, MODE, true>(a, P, lam_in, lam_out, gF, uint32_t(a.link_cell[b]), b);
+    return;
+  }
+  const uint32_t tile = xcd_swizzle(bid - uint32_t(a.nlb), uint32_t(a.ndb));
+  const uint32_t c = tile * kAdjBlock + threadIdx.x;
+  if (c >= a.N) return;
+  if (a.flags && (a.flags[c] & 2)) return;
+  adjoint_cell<S, COLL, MODE, false>(a, P, lam_in, lam_out, gF, c, -1);
+}
+
+// Prologue transpose: dL/dS_0 = ([pre guard] collide bounce)^T lambda_0, cell-local.
+template <class S, int COLL>
+__global__ __launch_bounds__(kAdjBlock) void k_adjoint_collide(AdjArgs<typename S::R> a,
+                                                               const typename S::T* __restrict__ S0,
+                                                               const typename S::R* __restrict__ lam_in,
+                                                               typename S::R* __restrict__ g_out) {
+  using R = typename S::R;
+  const uint32_t c = blockIdx.x * kAdjBlock + threadIdx.x;
+  if (c >= a.N) return;
+  const size_t N = a.N;
+  R s[Q], g[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) s[i] = S::ld_c(S0, i * N + c, a.off[i]);
+  bool pre_reset = false;
+  if (a.guard == LBM_GUARD_PRE && guard_bad(s)) {
+    pre_reset = true;
+    const R uu = a.uref[0] * a.uref[0] + a.uref[1] * a.uref[1] + a.uref[2] * a.uref[2];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) s[i] = feq(i, R(1), a.uref[0], a.uref[1], a.uref[2], uu);
+  }
+#pragma unroll
+  for (int i = 0; i < Q; ++i) g[i] = lam_in[i * N + c];
+  if (a.bounce == LBM_BOUNCE_HARD && (a.flags[c] & 1)) {
+#pragma unroll
+    for (int i = 1; i < Q; ++i) {
+      if (opp(i) > i) {
+        const R t = g[i];
+        g[i] = g[opp(i)];
+        g[opp(i)] = t;
+      }
+    }
+  } else if (a.bounce == LBM_BOUNCE_SOFT) {
+    const R p = a.phi[c];
+    R cc[Q];
+    const R tau_e = a.smag ? smag_tau(s, a.tau, a.cs) : a.tau;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) cc[i] = s[i];
+    if (COLL == LBM_COLL_MRT) collide_mrt(cc, R(1) / tau_e);
+    else collide_bgk(cc, R(1) / tau_e);
+    R gphi = R(0);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) gphi += g[i] * (cc[opp(i)] - cc[i]);
+    if (a.phi_acc) a.phi_acc[c] += double(gphi);
+    R t[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) t[i] = g[i];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) g[i] = (R(1) - p) * t[i] + p * t[opp(i)];
+  }
+  if (pre_reset) {
+    // the reset state does not depend on S_0
+#pragma unroll
+    for (int i = 0; i < Q; ++i) g[i] = R(0);
+  } else {
+    const R tau_e = a.smag ? smag_tau(s, a.tau, a.cs) : a.tau;
+    const R omega = R(1) / tau_e;
+    const R gom = (COLL == LBM_COLL_MRT) ? mrt_vjp(s, g, omega) : bgk_vjp(s, g, omega);
+    if (a.smag) smag_vjp(s, g, a.tau, a.cs, -gom * omega * omega);
+  }
+#pragma unroll
+  for (int i = 0; i < Q; ++i) g_out[i * N + c] = g[i];
+}
+
+// Deterministic additions: lam[tgt[k]] += sum_c coef[c] * mu_link[src[c]]
+// (+ gF . e_j for force terms, encoded src = -1 - j).
+template <typename R>
+__global__ void k_adjoint_scatter(int n_targets, const int32_t* __restrict__ tgt, const int32_t* __restrict__ start,
+                                  const int32_t* __restrict__ src, const double* __restrict__ coef,
+                                  const R* __restrict__ mu_link, const double* __restrict__ gF, R* __restrict__ lam) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_targets) return;
+  R acc = R(0);
+  for (int e = start[k]; e < start[k + 1]; ++e) {
+    const int s = src[e];
+    if (s >= 0) {
+      acc += R(coef[e]) * mu_link[s];
+    } else if (gF) {
+      const int j = -1 - s;
+      R v = R(0);
+      const R gx = R(gF[0]), gy = R(gF[1]), gz = R(gF[2]);
+      switch (j) {  // e_j . gF
+        default: break;
+#define LBM_FORCE_CASE(J) case J: v = R(EX[J]) * gx + R(EY[J]) * gy + R(EZ[J]) * gz; break;
+        LBM_FORCE_CASE(1) LBM_FORCE_CASE(2) LBM_FORCE_CASE(3) LBM_FORCE_CASE(4) LBM_FORCE_CASE(5)
+        LBM_FORCE_CASE(6) LBM_FORCE_CASE(7) LBM_FORCE_CASE(8) LBM_FORCE_CASE(9) LBM_FORCE_CASE(10)
+        LBM_FORCE_CASE(11) LBM_FORCE_CASE(12) LBM_FORCE_CASE(13) LBM_FORCE_CASE(14) LBM_FORCE_CASE(15)
+        LBM_FORCE_CASE(16) LBM_FORCE_CASE(17) LBM_FORCE_CASE(18)
+#undef LBM_FORCE_CASE
+      }
+      acc += v;
+    }
+  }
+  lam[tgt[k]] += acc;
+}
+
+template <typename R>
+int make_adj_args(const lbm_step_desc* d, const lbm_adjoint_desc* ad, int mode, AdjArgs<R>& a) {
+  if (!d || !ad || d->nx < 2 || d->ny <= 0 || d->nz <= 0) return LBM_E_ARG;
+  const long long n = 1LL * d->nx * d->ny * d->nz;
+  if (n * Q >= (1LL << 31)) return LBM_E_ARG;
+  if (!d->periodic_yz) return LBM_E_ARG;  // the owner-write scheme needs periodic y/z
+  a.nx = d->nx;
+  a.ny = d->ny;
+  a.nz = d->nz;
+  a.N = uint32_t(n);
+  a.dnx = make_fastdiv(uint32_t(d->nx));
+  a.dny = make_fastdiv(uint32_t(d->ny));
+  a.smag = d->smagorinsky;
+  a.bounce = d->bounce;
+  a.guard = d->guard;
+  a.n_links = d->n_links;
+  a.tau = R(d->tau);
+  a.cs = R(d->smag_cs);
+  for (int k = 0; k < 3; ++k) {
+    a.uin[k] = R(d->u_in[k]);
+    a.uref[k] = R(d->u_ref[k]);
+  }
+  a.rho_out = R(d->rho_out);
+  for (int i = 0; i < Q; ++i) a.off[i] = R(d->offset[i]);
+  a.flags = d->flags;
+  a.phi = static_cast<const R*>(d->porosity);
+  a.link_cell = d->link_cell;
+  a.link_q = d->link_q;
+  a.dq_acc = ad->dq_acc;
+  a.mu_link = static_cast<R*>(ad->mu_link);
+  a.phi_acc = ad->phi_acc;
+  a.u_acc = ad->u_acc;
+  if (a.bounce == LBM_BOUNCE_HARD && !a.flags) return LBM_E_ARG;
+  if (a.bounce == LBM_BOUNCE_SOFT && !a.phi) return LBM_E_ARG;
+  if (a.n_links < 0 || (a.n_links > 0 && (!a.link_cell || !a.link_q || !a.flags || !a.mu_link))) return LBM_E_ARG;
+  a.nlb = (mode == LBM_MODE_COLLIDE) ? 0 : (a.n_links + kAdjBlock - 1) / kAdjBlock;
+  a.ndb = int((n + kAdjBlock - 1) / kAdjBlock);
+  return LBM_OK;
+}
+
+template <class S>
+int launch_adjoint(const lbm_step_desc* d, const lbm_adjoint_desc* ad, int mode, const void* state,
+                   const void* lam_in, void* lam_out, const double* gF, hipStream_t st) {
+  using R = typename S::R;
+  AdjArgs<R> a;
+  const int rc = make_adj_args(d, ad, mode, a);
+  if (rc) return rc;
+  if (!state || !lam_in || !lam_out || lam_in == lam_out) return LBM_E_ARG;
+  const auto* P = static_cast<const typename S::T*>(state);
+  const auto* li = static_cast<const R*>(lam_in);
+  auto* lo = static_cast<R*>(lam_out);
+  const bool mrt = d->collision == LBM_COLL_MRT;
+  if (mode == LBM_MODE_COLLIDE) {
+    const unsigned grid = unsigned(a.ndb);
+    if (mrt) hipLaunchKernelGGL((k_adjoint_collide<S, LBM_COLL_MRT>), grid, kAdjBlock, 0, st, a, P, li, lo);
+    else hipLaunchKernelGGL((k_adjoint_collide<S, LBM_COLL_BGK>), grid, kAdjBlock, 0, st, a, P, li, lo);
+  } else {
+    const unsigned grid = unsigned(a.nlb + a.ndb);
+#define LBM_ADJ(COLL, MODE) hipLaunchKernelGGL((k_adjoint<S, COLL, MODE>), grid, kAdjBlock, 0, st, a, P, li, lo, gF)
+    if (mode == LBM_MODE_FULL) {
+      if (mrt) LBM_ADJ(LBM_COLL_MRT, LBM_MODE_FULL);
+      else LBM_ADJ(LBM_COLL_BGK, LBM_MODE_FULL);
+    } else if (mode == LBM_MODE_STREAM) {
+      if (mrt) LBM_ADJ(LBM_COLL_MRT, LBM_MODE_STREAM);
+      else LBM_ADJ(LBM_COLL_BGK, LBM_MODE_STREAM);
+    } else {
+      return LBM_E_ARG;
+    }
+#undef LBM_ADJ
+  }
+  return hipGetLastError() == hipSuccess ? LBM_OK : LBM_E_LAUNCH;
+}
+
+}  // namespace lbm
+
+using namespace lbm;
+
+extern "C" {
+
+int lbm_step_adjoint(const lbm_step_desc* d, const lbm_adjoint_desc* ad, int mode, const void* state,
+                     const void* lam_in, void* lam_out, const double* gF, lbm_stream_t stream) {
+  if (!d) return LBM_E_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  switch (d->storage) {
+    case LBM_F32: return launch_adjoint<StoreF32>(d, ad, mode, state, lam_in, lam_out, gF, s);
+    case LBM_F64: return launch_adjoint<StoreF64>(d, ad, mode, state, lam_in, lam_out, gF, s);
+    case LBM_F16DEV: return launch_adjoint<StoreF16>(d, ad, mode, state, lam_in, lam_out, gF, s);
+    case LBM_BF16DEV: return launch_adjoint<StoreBF16>(d, ad, mode, state, lam_in, lam_out, gF, s);
+  }
+  return LBM_E_DTYPE;
+}
+
+int lbm_adjoint_scatter(const lbm_step_desc* d, const lbm_adjoint_desc* ad, void* lam, const double* gF,
+                        lbm_stream_t stream) {
+  if (!d || !ad || !lam) return LBM_E_ARG;
+  if (ad->n_targets <= 0) return LBM_OK;
+  if (!ad->tgt_index || !ad->tgt_start || !ad->src || !ad->coef || !ad->mu_link) return LBM_E_ARG;
+  const unsigned grid = unsigned((ad->n_targets + 255) / 256);
+  hipStream_t s = (hipStream_t)stream;
+  if (d->storage == LBM_F64) {
+    hipLaunchKernelGGL(k_adjoint_scatter<double>, grid, 256, 0, s, ad->n_targets, ad->tgt_index, ad->tgt_start,
+                       ad->src, ad->coef, static_cast<const double*>(ad->mu_link), gF, static_cast<double*>(lam));
+  } else {
+    hipLaunchKernelGGL(k_adjoint_scatter<float>, grid, 256, 0, s, ad->n_targets, ad->tgt_index, ad->tgt_start,
+                       ad->src, ad->coef, static_cast<const float*>(ad->mu_link), gF, static_cast<float*>(lam));
+  }
+  return hipGetLastError() == hipSuccess ? LBM_OK : LBM_E_LAUNCH;
+}
+
+}  // extern "C"

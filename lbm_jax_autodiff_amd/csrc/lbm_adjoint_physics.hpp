@@ -1,0 +1,233 @@
+// Cell-local reverse-mode (vector-Jacobian) products of the D3Q19 step.
+//
+// Each function takes the forward inputs of one operator and a cotangent g
+// of its outputs, and overwrites g with the cotangent of its inputs.  The
+// conventions follow JAX's transposition of the reference graph: a `where`
+// sends the cotangent to the taken branch only, comparisons (guards, masks)
+// carry none, and sqrt at 0 uses the subgradient 0 (JAX would give NaN there;
+// this is the one documented deviation).
+#pragma once
+
+#include "lbm_physics.hpp"
+
+namespace lbm {
+
+// Pull back (G_rho|u, G_u) -- the cotangents of rho and u at fixed other
+// variable -- to the populations through rho = sum f, u = j / rho:
+//   g_i += G_rho - (G_u . u)/rho + (G_u . e_i)/rho.
+template <typename R>
+__device__ __forceinline__ void add_macro_pullback(R* g, const Macro<R>& m, R Grho, R Gux, R Guy, R Guz) {
+  const R inv = R(1) / m.rho;
+  const R a = Grho - (Gux * m.ux + Guy * m.uy + Guz * m.uz) * inv;
+  const R bx = Gux * inv, by = Guy * inv, bz = Guz * inv;
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    R v = a;
+    if (EX[i]) v += EX[i] > 0 ? bx : -bx;
+    if (EY[i]) v += EY[i] > 0 ? by : -by;
+    if (EZ[i]) v += EZ[i] > 0 ? bz : -bz;
+    g[i] += v;
+  }
+}
+
+// Cotangents of (rho, u) from a cotangent h on f_eq(rho, u):
+//   d feq_k / d rho|u = feq_k / rho,
+//   d feq_k / d u_a   = w_k rho (3 e_ka + 9 (e_k.u) e_ka - 3 u_a).
+template <typename R>
+__device__ __forceinline__ void feq_vjp(const R* h, const Macro<R>& m, R& Grho, R& Gux, R& Guy, R& Guz) {
+  Grho = 0;
+  Gux = 0;
+  Guy = 0;
+  Guz = 0;
+  R hw = 0;  // sum_k h_k w_k
+#pragma unroll
+  for (int k = 0; k < Q; ++k) {
+    R eu = R(0);
+    if (EX[k]) eu += EX[k] > 0 ? m.ux : -m.ux;
+    if (EY[k]) eu += EY[k] > 0 ? m.uy : -m.uy;
+    if (EZ[k]) eu += EZ[k] > 0 ? m.uz : -m.uz;
+    const R wk = R(weight(k));
+    const R hk = h[k] * wk;
+    Grho += hk * (R(1) + R(3) * eu + R(4.5) * eu * eu - R(1.5) * m.usq);
+    hw += hk;
+    const R t = hk * (R(3) + R(9) * eu);
+    if (EX[k]) Gux += EX[k] > 0 ? t : -t;
+    if (EY[k]) Guy += EY[k] > 0 ? t : -t;
+    if (EZ[k]) Guz += EZ[k] > 0 ? t : -t;
+  }
+  Gux = m.rho * (Gux - R(3) * m.ux * hw);
+  Guy = m.rho * (Guy - R(3) * m.uy * hw);
+  Guz = m.rho * (Guz - R(3) * m.uz * hw);
+}
+
+// BGK: c = s - omega (s - feq(s)).  g <- J^T g; returns dL/domega.
+template <typename R>
+__device__ __forceinline__ R bgk_vjp(const R* s, R* g, R omega) {
+  const Macro<R> m = macro(s);
+  R h[Q];
+  R gom = 0;
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const R e = feq(i, m.rho, m.ux, m.uy, m.uz, m.usq);
+    gom -= g[i] * (s[i] - e);
+    h[i] = omega * g[i];
+    g[i] = (R(1) - omega) * g[i];
+  }
+  R Gr, Gx, Gy, Gz;
+  feq_vjp(h, m, Gr, Gx, Gy, Gz);
+  add_macro_pullback(g, m, Gr, Gx, Gy, Gz);
+  return gom;
+}
+
+// d meq_k / d rho|u and d meq_k / d u (non-conserved rows), contracted with z.
+template <typename R>
+__device__ __forceinline__ void meq_vjp(const R* z, const Macro<R>& m, R& Grho, R& Gux, R& Guy, R& Guz) {
+  const R r = m.rho, ux = m.ux, uy = m.uy, uz = m.uz, u2 = m.usq;
+  const R dxx = R(2) * ux * ux - uy * uy - uz * uz;
+  const R dww = uy * uy - uz * uz;
+  const R c23 = R(-2.0 / 3.0);
+  Grho = z[1] * (R(-11) + R(19) * u2) + z[2] * (R(3) - R(5.5) * u2) + c23 * (z[4] * ux + z[6] * uy + z[8] * uz) +
+         (z[9] - R(0.5) * z[10]) * dxx + (z[11] - R(0.5) * z[12]) * dww + z[13] * ux * uy + z[14] * uy * uz +
+         z[15] * ux * uz;
+  // rows 9/10 and 11/12 share their u-dependence up to the factor -1/2
+  const R a = z[9] - R(0.5) * z[10];
+  const R b = z[11] - R(0.5) * z[12];
+  const R e = R(38) * z[1] - R(11) * z[2];
+  Gux = r * (e * ux + c23 * z[4] + R(4) * a * ux + z[13] * uy + z[15] * uz);
+  Guy = r * (e * uy + c23 * z[6] - R(2) * a * uy + R(2) * b * uy + z[13] * ux + z[14] * uz);
+  Guz = r * (e * uz + c23 * z[8] - R(2) * a * uz - R(2) * b * uz + z[14] * uy + z[15] * ux);
+}
+
+// MRT: c = s - M^T D (M s - meq(s)), D = diag(rate_k / ||M_k||^2).
+// J^T g = g - M^T h + (d meq/d s)^T h with h = D M g.  Returns dL/domega.
+template <typename R>
+__device__ __forceinline__ R mrt_vjp(const R* s, R* g, R omega) {
+  const Macro<R> mac = macro(s);
+  R h[Q];
+  R gom = 0;
+#pragma unroll
+  for (int k = 0; k < Q; ++k) {
+    if (mrt_is_conserved(k)) {
+      h[k] = R(0);
+      continue;
+    }
+    const R Mg = moment(k, g);
+    const R inv_n = R(1.0 / mrt_norm(k));
+    if (mrt_is_stress(k)) {
+      const R dm = moment(k, s) - meq(k, mac.rho, mac.ux, mac.uy, mac.uz, mac.usq);
+      gom -= Mg * dm * inv_n;
+      h[k] = omega * Mg * inv_n;
+    } else {
+      h[k] = R(mrt_rate_fixed(k)) * Mg * inv_n;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    R acc = R(0);
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+      if (mrt_is_conserved(k)) continue;
+      const int c = mrt_m(k, i);
+      if (c == 1) acc += h[k];
+      else if (c == -1) acc -= h[k];
+      else if (c != 0) acc += R(c) * h[k];
+    }
+    g[i] -= acc;
+  }
+  R Gr, Gx, Gy, Gz;
+  meq_vjp(h, mac, Gr, Gx, Gy, Gz);
+  add_macro_pullback(g, mac, Gr, Gx, Gy, Gz);
+  return gom;
+}
+
+// Smagorinsky: tau_eff(s) = (tau + sqrt(tau^2 + 18 cs^2 |Pi| / (rho + 1e-10))) / 2,
+// Pi_ab = sum_i e_ia e_ib (s_i - feq_i(s)).  Adds g_tau_eff * d tau_eff / d s to g.
+template <typename R>
+__device__ __forceinline__ void smag_vjp(const R* s, R* g, R tau, R cs, R g_teff) {
+  const Macro<R> m = macro(s);
+  R sxx = 0, syy = 0, szz = 0, sxy = 0, sxz = 0, syz = 0;
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const R n = s[i] - feq(i, m.rho, m.ux, m.uy, m.uz, m.usq);
+    if (EX[i]) sxx += n;
+    if (EY[i]) syy += n;
+    if (EZ[i]) szz += n;
+    if (EX[i] * EY[i]) sxy += EX[i] * EY[i] > 0 ? n : -n;
+    if (EX[i] * EZ[i]) sxz += EX[i] * EZ[i] > 0 ? n : -n;
+    if (EY[i] * EZ[i]) syz += EY[i] * EZ[i] > 0 ? n : -n;
+  }
+  const R pi = sqrt(sxx * sxx + syy * syy + szz * szz + R(2) * (sxy * sxy + sxz * sxz + syz * syz));
+  const R den = m.rho + R(1e-10);
+  const R k18 = R(18) * cs * cs;
+  const R root = sqrt(tau * tau + k18 * pi / den);
+  // d tau_eff / d pi and d tau_eff / d rho (explicit)
+  const R dpi = R(0.25) * k18 / (den * root);
+  const R drho = R(-0.25) * k18 * pi / (den * den * root);
+  const R gpi = g_teff * dpi;
+  // subgradient 0 at |Pi| = 0
+  const R inv = pi > R(0) ? gpi / pi : R(0);
+  const R gxx = inv * sxx, gyy = inv * syy, gzz = inv * szz;
+  const R gxy = R(2) * inv * sxy, gxz = R(2) * inv * sxz, gyz = R(2) * inv * syz;
+  // cotangent on the non-equilibrium part n_i = s_i - feq_i
+  R hn[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    R v = R(0);
+    if (EX[i]) v += gxx;
+    if (EY[i]) v += gyy;
+    if (EZ[i]) v += gzz;
+    if (EX[i] * EY[i]) v += EX[i] * EY[i] > 0 ? gxy : -gxy;
+    if (EX[i] * EZ[i]) v += EX[i] * EZ[i] > 0 ? gxz : -gxz;
+    if (EY[i] * EZ[i]) v += EY[i] * EZ[i] > 0 ? gyz : -gyz;
+    hn[i] = v;
+    g[i] += v;
+  }
+  R Gr, Gx, Gy, Gz;
+  feq_vjp(hn, m, Gr, Gx, Gy, Gz);
+  const R grho_direct = g_teff * drho;
+  add_macro_pullback(g, m, grho_direct - Gr, -Gx, -Gy, -Gz);
+}
+
+// Zou-He inlet transpose.  s holds the pulled (pre-inlet) values; g the
+// cotangent of the post-inlet values.  Accumulates dL/du_in into gu.
+template <typename R>
+__device__ __forceinline__ void zou_he_in_vjp(const R* s, R* g, R ux, R uy, R uz, double* gu) {
+  const R k0 = s[0] + s[3] + s[4] + s[5] + s[6] + s[15] + s[16] + s[17] + s[18];
+  const R km = s[2] + s[8] + s[10] + s[12] + s[14];
+  const R rho = (k0 + R(2) * km) / (R(1) - ux);
+  const R g1 = g[1], g7 = g[7], g9 = g[9], g11 = g[11], g13 = g[13];
+  const R grho = g1 * ux * R(1.0 / 3.0) + (g7 * (ux + uy) + g9 * (ux - uy) + g11 * (ux + uz) + g13 * (ux - uz)) *
+                 R(1.0 / 6.0);
+  if (gu) {
+    gu[0] += double(g1 * rho * R(1.0 / 3.0) + (g7 + g9 + g11 + g13) * rho * R(1.0 / 6.0) + grho * rho / (R(1) - ux));
+    gu[1] += double((g7 - g9) * rho * R(1.0 / 6.0));
+    gu[2] += double((g11 - g13) * rho * R(1.0 / 6.0));
+  }
+  const R a = grho / (R(1) - ux);
+  g[1] = g[7] = g[9] = g[11] = g[13] = R(0);
+  g[0] += a; g[3] += a; g[4] += a; g[5] += a; g[6] += a;
+  g[15] += a; g[16] += a; g[17] += a; g[18] += a;
+  g[2] += R(2) * a + g1;
+  g[10] += R(2) * a + g7;
+  g[8] += R(2) * a + g9;
+  g[14] += R(2) * a + g11;
+  g[12] += R(2) * a + g13;
+}
+
+// Zou-He outlet transpose (u_x = -1 + (K0 + 2 K+) / rho_out).
+template <typename R>
+__device__ __forceinline__ void zou_he_out_vjp(R* g, R rho_out) {
+  const R g2 = g[2], g10 = g[10], g8 = g[8], g14 = g[14], g12 = g[12];
+  const R gux = -rho_out * (g2 * R(1.0 / 3.0) + (g10 + g8 + g14 + g12) * R(1.0 / 6.0));
+  const R a = gux / rho_out;
+  g[2] = g[10] = g[8] = g[14] = g[12] = R(0);
+  g[0] += a; g[3] += a; g[4] += a; g[5] += a; g[6] += a;
+  g[15] += a; g[16] += a; g[17] += a; g[18] += a;
+  g[1] += R(2) * a + g2;
+  g[7] += R(2) * a + g10;
+  g[9] += R(2) * a + g8;
+  g[11] += R(2) * a + g14;
+  g[13] += R(2) * a + g12;
+}
+
+}  // namespace lbm

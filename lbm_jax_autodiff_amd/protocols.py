@@ -1,0 +1,272 @@
+"""Run protocols of the reference's drivers on the fused HIP engine.
+
+* channel flow         examples/01_channel_flow.py:22-89
+* sphere Cd            examples/10_cd_convergence.py:46-121 (time-averaged Cd)
+* sphere dCd/dr, Adam  examples/11_optimisation_v2.py:56-204
+* sphere geometry at a world centre on the box [-4,4]x[-2,2]x[-2,2]; a centre
+  given in lattice units is read as the continuous coordinate X (node i at
+  X = i + 1/2), i.e. world = X / s - (4, 2, 2) with s = nx/8.
+
+Usage: python -m lbm_jax_autodiff_amd.protocols {c1,c2,c3,c4,c5} [options]
+prints one JSON line per result.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import sys
+import time
+from typing import Optional
+
+import torch
+
+from . import _field as F
+from .adjoint import drag_series, rollout
+from .core.ops import compute_density, compute_velocity, equilibrium, projected_area
+from .engine import Lattice
+from .geometry import compute_q_from_solid, create_sphere, smooth_sphere_geometry
+
+
+def _dev():
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _dtype(storage):
+    return torch.float64 if storage == "f64" else torch.float32
+
+
+def world_center(nx, ny, nz, center_lattice):
+    return (center_lattice[0] / (nx / 8.0) - 4.0, center_lattice[1] / (ny / 4.0) - 2.0,
+            center_lattice[2] / (nz / 4.0) - 2.0)
+
+
+def tau_for(u, D, re):
+    return 3.0 * (u * D / re) + 0.5
+
+
+# ---------------------------------------------------------------- channel --
+
+def channel_flow(nx=64, ny=21, nz=21, tau=0.8, u_in=0.05, n_steps=5000, storage="f32"):
+    """Poiseuille channel: halfway walls at y = 0, ny-1, BGK, Zou-He, f0 = w, no guard."""
+    dev = _dev()
+    dt = _dtype(storage)
+    solid = torch.zeros((nx, ny, nz), dtype=torch.int32, device=dev)
+    solid[:, 0, :] = 1
+    solid[:, ny - 1, :] = 1
+    q = compute_q_from_solid(solid, nx, ny, nz, 0.5, dtype=dt)
+    f0 = equilibrium(torch.ones((nx, ny, nz), dtype=dt, device=dev), torch.zeros((3, nx, ny, nz), dtype=dt,
+                                                                                  device=dev))
+    lat = Lattice(nx, ny, nz, tau=tau, inlet_velocity=(u_in, 0.0, 0.0), collision="bgk", solid=solid, q=q,
+                  storage=storage, device=dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s, _, _ = lat.run(f0, n_steps, forces=False)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    s = s.to(torch.float64)
+    rho = compute_density(s)
+    ux = compute_velocity(s, rho)[0]
+    prof = ux[nx // 2].mean(-1).cpu()
+    H = float(ny - 2)
+    umax = float(prof[1:-1].max())
+    ana = torch.tensor([0.0 if j in (0, ny - 1) else umax * 4.0 * (j - 0.5) * (H - (j - 0.5)) / (H * H)
+                        for j in range(ny)], dtype=torch.float64)
+    fluid = slice(1, ny - 1)
+    l2 = float(torch.linalg.norm(prof[fluid] - ana[fluid]) / torch.linalg.norm(ana[fluid]))
+    return {"config": "channel", "grid": [nx, ny, nz], "steps": n_steps, "storage": storage, "tau": tau,
+            "profile": [round(float(v), 8) for v in prof], "rel_l2_vs_parabola": l2,
+            "mlups": nx * ny * nz * n_steps / wall / 1e6, "wall_s": wall}
+
+
+# ----------------------------------------------------------------- sphere --
+
+def sphere_lattice(nx, ny, nz, center_world, radius=0.5, *, re=100.0, u_in=0.05, collision="mrt",
+                   mode="hard", sharpness=10.0, smagorinsky_cs=None, guard="pre", storage="f32",
+                   offset="inlet"):
+    dev = _dev()
+    D = 2.0 * radius * nx / 8.0
+    tau = tau_for(u_in, D, re)
+    if mode == "hard":
+        solid, q = create_sphere(nx, ny, nz, center_world, radius, device=dev)
+        kw = dict(solid=solid, q=q.to(_dtype(storage)))
+    else:
+        por, solid, q = smooth_sphere_geometry(nx, ny, nz, center_world, radius, sharpness, device=dev,
+                                               out_dtype=_dtype(storage))
+        kw = dict(porosity=por, q=q)
+    lat = Lattice(nx, ny, nz, tau=tau, inlet_velocity=(u_in, 0.0, 0.0), collision=collision,
+                  smagorinsky_cs=smagorinsky_cs, guard=guard, guard_velocity=(u_in, 0.0, 0.0), storage=storage,
+                  offset=offset, device=dev, **kw)
+    area = float(projected_area(solid, 0))
+    return lat, solid, tau, area
+
+
+def tail_stats(cd, frac=0.3):
+    cd = cd.to(torch.float64)
+    n = cd.numel()
+    k = max(1, int(round(frac * n)))
+    tail = cd[n - k:]
+    return float(tail.mean()), float(tail.std(unbiased=False))
+
+
+def snapshot_stats(cd, stride=500, frac=0.3):
+    """examples/10_cd_convergence.py:88-106: a sample every `stride` steps,
+    mean and population std of the last max(3, int(0.3 * #samples))."""
+    samples = cd[stride - 1::stride].to(torch.float64).cpu()
+    if samples.numel() == 0:
+        return float("nan"), float("nan")
+    k = max(3, int(samples.numel() * frac))
+    tail = samples[-k:]
+    return float(tail.mean()), float(tail.std(unbiased=False))
+
+
+def sphere_cd(nx, ny, nz, n_steps, center_world, radius=0.5, *, init_u=0.0, storage="f32", **kw):
+    lat, solid, tau, area = sphere_lattice(nx, ny, nz, center_world, radius, storage=storage, **kw)
+    dev = lat.device
+    dt = _dtype(storage)
+    u = torch.zeros((3, nx, ny, nz), dtype=dt, device=dev)
+    u[0] = init_u
+    f0 = equilibrium(torch.ones((nx, ny, nz), dtype=dt, device=dev), u)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, _, forces = lat.run(f0, n_steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    u_in = lat.inlet_velocity[0]
+    cd = drag_series(forces, u_in, area)
+    mean, std = tail_stats(cd)
+    smean, sstd = snapshot_stats(cd)
+    return {"grid": [nx, ny, nz], "steps": n_steps, "storage": storage, "tau": tau, "area": area,
+            "center_world": list(center_world), "radius_world": radius, "links": lat.links.n,
+            "cd_tail30_mean": mean, "cd_tail30_std": std, "cd_snap500_mean": smean, "cd_snap500_std": sstd,
+            "cd_last": float(cd[-1]), "mlups": nx * ny * nz * n_steps / wall / 1e6, "wall_s": wall}, cd
+
+
+def sphere_loss_and_grad(nx, ny, nz, n_steps, center_world, radius=0.5, *, avg_last=600, q_mode="bouzidi",
+                         porosity_mode="hard", sharpness=20.0, collision="mrt", re=100.0, u_in=0.05,
+                         storage="f32", checkpoint="all", smagorinsky_cs=None, guard="none", init_u=0.0,
+                         offset="inlet", geometry_dtype=None):
+    """loss = mean Cd over the last `avg_last` steps and dloss/dr (world units)
+    through q(r) (Bouzidi) and porosity(r) (soft), examples/11:56-167."""
+    dev = _dev()
+    dt = _dtype(storage)
+    gdt = geometry_dtype or (torch.float64 if storage == "f64" else torch.float32)
+    r = torch.tensor(float(radius), dtype=gdt, device=dev, requires_grad=True)
+    por_soft, solid, q = smooth_sphere_geometry(nx, ny, nz, center_world, r, sharpness, device=dev,
+                                                coord_dtype=gdt, out_dtype=dt)
+    if q_mode == "halfway":
+        q = compute_q_from_solid(solid, nx, ny, nz, 0.5, dtype=dt)
+    porosity = por_soft if porosity_mode == "soft" else solid.to(dt) + 0.0 * por_soft
+    D = 2.0 * float(radius) * nx / 8.0
+    tau = tau_for(u_in, D, re)
+    u = torch.zeros((3, nx, ny, nz), dtype=dt, device=dev)
+    u[0] = init_u
+    f0 = equilibrium(torch.ones((nx, ny, nz), dtype=dt, device=dev), u)
+    area = torch.clamp(projected_area(solid, 0).to(torch.float64), min=1.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, _, forces = rollout(f0, n_steps, tau=tau, q=q, porosity=porosity, inlet_velocity=(u_in, 0.0, 0.0),
+                           collision=collision, smagorinsky_cs=smagorinsky_cs, guard=guard,
+                           guard_velocity=(u_in, 0.0, 0.0), storage=storage, offset=offset, checkpoint=checkpoint)
+    cd = drag_series(forces, u_in, area)
+    loss = cd[-avg_last:].mean()
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    (g,) = torch.autograd.grad(loss, r, allow_unused=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    g = 0.0 if g is None else float(g)
+    return {"grid": [nx, ny, nz], "steps": n_steps, "storage": storage, "q_mode": q_mode,
+            "porosity_mode": porosity_mode, "tau": tau, "radius_world": float(radius), "loss": float(loss),
+            "dloss_dr_world": g, "dloss_dr_lattice": g / (nx / 8.0), "checkpoint": str(checkpoint),
+            "fwd_s": t_fwd, "fwd_adj_s": wall, "fwd_adj_mlups": nx * ny * nz * n_steps / wall / 1e6}
+
+
+def adam(loss_and_grad, r0, n_iters, lr, r_min=0.3, r_max=0.8, b1=0.9, b2=0.999, eps=1e-8):
+    """optax.adam(lr) + clip to [r_min, r_max] on a scalar (examples/11:182-194)."""
+    r, m, v = float(r0), 0.0, 0.0
+    hist = []
+    for t in range(1, n_iters + 1):
+        loss, g = loss_and_grad(r)
+        hist.append({"iter": t - 1, "r": r, "loss": loss, "grad": g})
+        m = b1 * m + (1.0 - b1) * g
+        v = b2 * v + (1.0 - b2) * g * g
+        mh = m / (1.0 - b1 ** t)
+        vh = v / (1.0 - b2 ** t)
+        r = r - lr * mh / (math.sqrt(vh) + eps)
+        r = min(max(r, r_min), r_max)
+    return r, hist
+
+
+# -------------------------------------------------------------------- CLI --
+
+def _emit(d):
+    print(json.dumps(d), flush=True)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("config", choices=["c1", "c2", "c3", "c3fd", "c3adam", "c4", "c5"])
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--storage", default="f32")
+    ap.add_argument("--checkpoint", default=None)
+    ap.add_argument("--mode", default="hard")
+    args = ap.parse_args(argv)
+    if args.config == "c1":
+        _emit(channel_flow(n_steps=args.steps or 5000, storage=args.storage))
+    elif args.config == "c2":
+        nx, ny, nz = 64, 32, 32
+        res, _ = sphere_cd(nx, ny, nz, args.steps or 8000, world_center(nx, ny, nz, (16, 16, 16)), 0.5,
+                           storage=args.storage)
+        res["config"] = "c2"
+        _emit(res)
+    elif args.config in ("c3", "c3fd"):
+        nx, ny, nz = 96, 48, 48
+        c = world_center(nx, ny, nz, (24, 24, 24))
+        steps = args.steps or 2000
+        res = sphere_loss_and_grad(nx, ny, nz, steps, c, 0.5, storage=args.storage)
+        res["config"] = "c3"
+        _emit(res)
+        if args.config == "c3fd":
+            h = 1e-3
+            lp = sphere_loss_and_grad(nx, ny, nz, steps, c, 0.5 + h, storage="f64")["loss"]
+            lm = sphere_loss_and_grad(nx, ny, nz, steps, c, 0.5 - h, storage="f64")["loss"]
+            g64 = sphere_loss_and_grad(nx, ny, nz, steps, c, 0.5, storage="f64")
+            _emit({"config": "c3", "fd_f64_h1e-3": (lp - lm) / (2 * h), "adjoint_f64": g64["dloss_dr_world"],
+                   "adjoint_" + args.storage: res["dloss_dr_world"]})
+            hw = sphere_loss_and_grad(nx, ny, nz, steps, c, 0.5, storage=args.storage, q_mode="halfway")
+            hw["config"] = "c3-halfway-hard"
+            _emit(hw)
+    elif args.config == "c3adam":
+        nx, ny, nz = 96, 48, 48
+        c = world_center(nx, ny, nz, (24, 24, 24))
+        steps = args.steps or 2000
+
+        def lg(r):
+            out = sphere_loss_and_grad(nx, ny, nz, steps, c, r, storage=args.storage)
+            return out["loss"], out["dloss_dr_world"]
+        r, hist = adam(lg, 0.5, 20, 0.05)
+        _emit({"config": "c3-adam", "lr": 0.05, "iters": 20, "final_r": r, "history": hist})
+    elif args.config == "c4":
+        nx, ny, nz = 256, 128, 128
+        steps = args.steps or 28000
+        ck = int(args.checkpoint) if args.checkpoint else 100
+        res = sphere_loss_and_grad(nx, ny, nz, steps, (0.0, 0.0, 0.0), 0.5, storage=args.storage,
+                                   checkpoint=ck, avg_last=int(0.3 * steps), guard="pre")
+        res["config"] = "c4"
+        _emit(res)
+    elif args.config == "c5":
+        nx, ny, nz = 384, 192, 192
+        steps = args.steps or 40000
+        soft = args.mode == "soft"
+        res = sphere_loss_and_grad(nx, ny, nz, steps, (0.0, 0.0, 0.0), 0.5, storage=args.storage,
+                                   checkpoint=(200, 14), avg_last=int(0.3 * steps), guard="pre",
+                                   porosity_mode="soft" if soft else "hard", sharpness=10.0,
+                                   smagorinsky_cs=0.1 if soft else None)
+        res["config"] = "c5-" + args.mode
+        _emit(res)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,130 @@
+"""Host logic of the adjoint, checked on the CPU.
+
+1. The segment-recompute checkpoint schedules yield every P_t once, in
+   reverse order, for store-all, one-level and two-level policies.
+2. The transpose of streaming decomposes into the kernel's owner writes
+   (entry (x - e_i)_i gets mu(x)_i for a normal pull, 0 at x-face fallbacks
+   and Bouzidi links) plus the scatter plan: checked against torch autograd
+   of the oracle's stream() on a small grid with random Bouzidi links.
+"""
+
+import types
+
+import numpy as np
+import pytest
+import torch
+
+from lbm_jax_autodiff_amd import adjoint as A
+from oracle import lbm_oracle as O
+from _helpers import random_q
+
+
+class _CountingLattice:
+    """Stand-in lattice whose state is a 1-element tensor holding t."""
+
+    def __init__(self):
+        self.steps = 0
+
+    def new_state(self):
+        return torch.zeros(1)
+
+    def step(self, src, dst):
+        self.steps += 1
+        dst.copy_(src + 1)
+
+
+@pytest.mark.parametrize("policy,n", [("all", 7), (3, 10), (4, 4), ((7, 3), 23), ((200, 14), 431), ((5, 5), 12)])
+def test_checkpoint_schedules(policy, n):
+    lat = _CountingLattice()
+    ck = A._Checkpoints(lat, policy, n)
+    cur = torch.zeros(1)
+    for t in range(n):
+        ck.record(t, cur)
+        if t < n - 1:
+            nxt = torch.zeros(1)
+            nxt.copy_(cur + 1)
+            cur = nxt
+    seen = []
+    for t, p in ck.descending():
+        assert float(p) == float(t)
+        seen.append(t)
+    assert seen == list(range(n - 1, -1, -1))
+
+
+def _owner_transpose(mu, q, nx, ny, nz):
+    """Kernel owner writes, emulated: lam(x - e_i)_i = mu(x)_i unless x keeps
+    its own value (x faces) or direction i is a Bouzidi link at x."""
+    lam = torch.zeros_like(mu)
+    for i in range(19):
+        ex, ey, ez = O.VELOCITIES[i]
+        for x in range(nx):
+            fallback = (ex == 1 and x == 0) or (ex == -1 and x == nx - 1)
+            link = (q[O.OPPOSITE[i], x] >= 0) if i else torch.zeros((ny, nz), dtype=torch.bool)
+            val = torch.where(link, torch.zeros_like(mu[i, x]), mu[i, x])
+            if fallback:
+                val = torch.zeros_like(val)
+            tx = (x - ex) % nx
+            lam[i, tx] = torch.roll(val, shifts=(-ey, -ez), dims=(0, 1))
+    return lam
+
+
+def test_stream_transpose_decomposition():
+    nx, ny, nz = 7, 5, 4
+    g = torch.Generator().manual_seed(0)
+    P = torch.rand((19, nx, ny, nz), generator=g, dtype=torch.float64, requires_grad=True)
+    q = random_q(nx, ny, nz, seed=3, p_wall=0.2)
+    S = O.stream(P, q, True)
+    mu = torch.randn(S.shape, generator=g, dtype=torch.float64)
+    # Zou-He overwrites the x-face fallback populations, so their cotangent is 0
+    for i in range(19):
+        ex = O.VELOCITIES[i][0]
+        if ex == 1:
+            mu[i, 0] = 0.0
+        if ex == -1:
+            mu[i, nx - 1] = 0.0
+    (ref,) = torch.autograd.grad(S, P, mu)
+
+    lam = _owner_transpose(mu, q, nx, ny, nz)
+    # scatter plan on a stand-in lattice carrying the compiled links
+    N = nx * ny * nz
+    qs = q.permute(0, 3, 2, 1).contiguous().reshape(19, N)  # lattice storage (x fastest)
+    has = (qs[1:] >= 0).any(0)
+    cells = torch.nonzero(has).reshape(-1)
+    links = types.SimpleNamespace(n=int(cells.numel()), cell=cells.to(torch.int32), q=qs[1:, cells].clone())
+    lat = types.SimpleNamespace(links=links, nx=nx, ny=ny, nz=nz, N=N, device=torch.device("cpu"))
+    tgt, start, src, coef = A.scatter_plan(lat)
+    mu_s = mu.permute(0, 3, 2, 1).contiguous().reshape(19, N)
+    mu_link = torch.zeros((19, links.n), dtype=torch.float64)
+    for i in range(1, 19):
+        link_i = qs[O.OPPOSITE[i], cells] >= 0
+        mu_link[i] = torch.where(link_i, mu_s[i, cells], torch.zeros(()))
+    lam_s = lam.permute(0, 3, 2, 1).contiguous().reshape(-1)
+    mu_flat = mu_link.reshape(-1)
+    for k in range(tgt.numel()):
+        acc = 0.0
+        for e in range(int(start[k]), int(start[k + 1])):
+            s = int(src[e])
+            if s >= 0:
+                acc += float(coef[e]) * float(mu_flat[s])
+        lam_s[int(tgt[k])] += acc
+    lam = lam_s.reshape(19, nz, ny, nx).permute(0, 3, 2, 1)
+    assert float((lam - ref).abs().max()) < 1e-13
+
+
+def test_scatter_plan_is_sorted_and_complete():
+    nx, ny, nz = 6, 5, 4
+    q = random_q(nx, ny, nz, seed=5, p_wall=0.3)
+    N = nx * ny * nz
+    qs = q.permute(0, 3, 2, 1).contiguous().reshape(19, N)
+    cells = torch.nonzero((qs[1:] >= 0).any(0)).reshape(-1)
+    links = types.SimpleNamespace(n=int(cells.numel()), cell=cells.to(torch.int32), q=qs[1:, cells].clone())
+    lat = types.SimpleNamespace(links=links, nx=nx, ny=ny, nz=nz, N=N, device=torch.device("cpu"))
+    tgt, start, src, coef = A.scatter_plan(lat)
+    t = tgt.numpy()
+    assert np.all(np.diff(t) > 0)
+    n_links = int((qs[1:, cells] >= 0).sum())
+    n_force = int((src < 0).sum())
+    assert n_force == n_links
+    n_hi = int((qs[1:, cells][qs[1:, cells] >= 0] >= 0.5).sum())
+    assert int(src.numel()) == n_force + n_links + n_links  # own + (own i | neighbour) + force
+    assert n_hi <= n_links

@@ -1,0 +1,171 @@
+"""Hand-written HIP adjoint against torch autograd of the FP64 oracle, the
+reference's published FP64 gradient KATs, finite differences and the exact
+zero of the halfway/hard-mask mode."""
+
+import itertools
+
+import pytest
+import torch
+
+from lbm_jax_autodiff_amd import adjoint as A
+import lbm_jax_autodiff_amd as lbm
+from oracle import lbm_oracle as O
+from _helpers import near_eq_state, random_q, cpu
+
+pytestmark = pytest.mark.gpu
+
+NX, NY, NZ = 12, 8, 6
+
+
+def _weights(seed, shape):
+    g = torch.Generator().manual_seed(seed)
+    return torch.randn(shape, generator=g, dtype=torch.float64)
+
+
+CASES = list(itertools.product(["bgk", "mrt"], ["none", "hard", "soft"], ["none", "post", "pre"], [False, True]))
+
+
+@pytest.mark.parametrize("collision,bounce,guard,smag", CASES)
+def test_adjoint_matches_oracle_autograd_f64(dev, collision, bounce, guard, smag):
+    steps = 5
+    f0 = near_eq_state(NX, NY, NZ, amp=2e-3)
+    q = random_q(NX, NY, NZ, seed=4, p_wall=0.1)
+    g = torch.Generator().manual_seed(2)
+    solid = (torch.rand((NX, NY, NZ), generator=g) < 0.1).to(torch.int32)
+    phi = 0.3 * torch.rand((NX, NY, NZ), generator=g, dtype=torch.float64)
+    wS, wP, wF = _weights(1, (19, NX, NY, NZ)), _weights(2, (19, NX, NY, NZ)), _weights(3, (steps, 3))
+    u0 = torch.tensor([0.05, 0.0, 0.0], dtype=torch.float64)
+
+    # oracle with torch autograd
+    f0r = f0.clone().requires_grad_(True)
+    qr = q.clone().requires_grad_(True)
+    phr = phi.clone().requires_grad_(True)
+    ur = u0.clone().requires_grad_(True)
+    feq_ref = O.equilibrium(torch.ones((NX, NY, NZ), dtype=torch.float64),
+                            torch.stack([torch.full((NX, NY, NZ), 0.05, dtype=torch.float64),
+                                         torch.zeros((NX, NY, NZ), dtype=torch.float64),
+                                         torch.zeros((NX, NY, NZ), dtype=torch.float64)]))
+    s_r, p_r, F_r = O.rollout(f0r, steps, forces=True, tau=torch.tensor(0.6, dtype=torch.float64), velocity=ur,
+                              q=qr, solid=solid if bounce == "hard" else None,
+                              porosity=phr if bounce == "soft" else None, collision=collision,
+                              smagorinsky_cs=0.1 if smag else None, guard=guard, f_eq_ref=feq_ref)
+    loss_r = (s_r * wS).sum() + (p_r * wP).sum() + (F_r * wF).sum()
+    inputs = [f0r, qr, ur] + ([phr] if bounce == "soft" else [])
+    grads_r = torch.autograd.grad(loss_r, inputs, allow_unused=True)
+
+    f0d = f0.to(dev).requires_grad_(True)
+    qd = q.to(dev).requires_grad_(True)
+    phd = phi.to(dev).requires_grad_(True)
+    ud = u0.to(dev).requires_grad_(True)
+    s, p, F = A.rollout(f0d, steps, tau=0.6, q=qd, porosity=phd if bounce == "soft" else None,
+                        solid=solid.to(dev) if bounce == "hard" else None, inlet_velocity=ud,
+                        collision=collision, smagorinsky_cs=0.1 if smag else None, guard=guard,
+                        guard_velocity=(0.05, 0.0, 0.0), storage="f64")
+    loss = (s * wS.to(dev)).sum() + (p * wP.to(dev)).sum() + (F * wF.to(dev)).sum()
+    inputs_d = [f0d, qd, ud] + ([phd] if bounce == "soft" else [])
+    grads = torch.autograd.grad(loss, inputs_d, allow_unused=True)
+    assert abs(float(loss) - float(loss_r)) < 1e-10 * max(1.0, abs(float(loss_r)))
+    for name, gd, gr in zip(["f0", "q", "u", "phi"], grads, grads_r):
+        gr = torch.zeros_like(cpu(gd)) if gr is None else gr
+        if name == "q":  # only link entries (q >= 0) carry a gradient
+            gr = torch.where(q >= 0, gr, torch.zeros_like(gr))
+        err = float((cpu(gd).to(torch.float64) - gr).abs().max())
+        scale = float(gr.abs().max()) + 1e-30
+        assert err < 1e-10 * max(1.0, scale), (name, err, scale)
+
+
+def test_kat_inlet_velocity_gradient(dev):
+    """FP64 d mean(rho[2:-2]) / dU after 5 BGK steps = 0.611238 (paper/paper.tex:502;
+    protocol tests/test_gradients_f64.py:95-127)."""
+    nx, ny, nz = 8, 4, 4
+    f0 = lbm.equilibrium(torch.ones((nx, ny, nz), dtype=torch.float64, device=dev),
+                         torch.zeros((3, nx, ny, nz), dtype=torch.float64, device=dev))
+    U = torch.tensor([0.05, 0.0, 0.0], dtype=torch.float64, device=dev, requires_grad=True)
+    s, _, _ = A.rollout(f0, 5, tau=0.8, q=None, inlet_velocity=U, collision="bgk", storage="f64")
+    loss = s[:, 2:-2].sum(0).mean()
+    (g,) = torch.autograd.grad(loss, U)
+    assert abs(float(g[0]) - 0.611238) < 5e-7
+
+
+def _kat_cd(r, dev):
+    """Cd(r) of tests/test_gradients_f64.py:129-207 on the GPU in FP64."""
+    nx, ny, nz = 32, 16, 16
+    u_in = 0.05
+    tau = 3.0 * (u_in * 8.0 / 50.0) + 0.5
+    por, solid, q = lbm.smooth_sphere_geometry(nx, ny, nz, (0.0, 0.0, 0.0), r, 20.0, coord_dtype=torch.float64,
+                                               out_dtype=torch.float64, disc_floor=1e-20, denom_eps=1e-30,
+                                               device=dev)
+    uu = torch.zeros((3, nx, ny, nz), dtype=torch.float64, device=dev)
+    uu[0] = u_in
+    f0 = lbm.equilibrium(torch.ones((nx, ny, nz), dtype=torch.float64, device=dev), uu)
+    _, _, F = A.rollout(f0, 10, tau=tau, q=q, porosity=por, inlet_velocity=(u_in, 0.0, 0.0), collision="bgk",
+                        storage="f64")
+    area = torch.clamp(lbm.projected_area(solid, 0).to(torch.float64), min=1.0)
+    return A.drag_series(F, u_in, area)[-1]
+
+
+def test_kat_bouzidi_radius_gradient(dev):
+    """FP64 Bouzidi dCd/dr over 10 steps = 93.33275 (paper/paper.tex:503)."""
+    r = torch.tensor(0.5, dtype=torch.float64, device=dev, requires_grad=True)
+    cd = _kat_cd(r, dev)
+    (g,) = torch.autograd.grad(cd, r)
+    assert abs(float(g) - 93.33275) < 5e-5
+    gref, cdref = O.kat_bouzidi_grad()
+    assert abs(float(cd) - cdref) < 1e-10
+    # central finite difference of the same FP64 kernels
+    h = 1e-6
+    fd = (float(_kat_cd(torch.tensor(0.5 + h, dtype=torch.float64, device=dev), dev))
+          - float(_kat_cd(torch.tensor(0.5 - h, dtype=torch.float64, device=dev), dev))) / (2 * h)
+    assert abs(fd - float(g)) / abs(float(g)) < 1e-6
+
+
+def test_halfway_hard_gradient_is_exactly_zero(dev):
+    """q = 1/2 and a hard mask make Cd independent of r: dCd/dr = 0.0 exactly
+    (README.md:35, paper/paper.tex:651-662)."""
+    nx, ny, nz = 32, 16, 16
+    r = torch.tensor(0.5, dtype=torch.float32, device=dev, requires_grad=True)
+    por, solid, _ = lbm.smooth_sphere_geometry(nx, ny, nz, (0.0, 0.0, 0.0), r, 20.0, device=dev)
+    q = lbm.compute_q_from_solid(solid, nx, ny, nz, 0.5)
+    # hard mode: porosity = solid (examples/11_optimisation_v2.py:62-65); the
+    # 0 * porosity(r) term keeps r on the graph so the whole adjoint runs
+    hard = solid.to(torch.float32) + 0.0 * por
+    uu = torch.zeros((3, nx, ny, nz), dtype=torch.float32, device=dev)
+    uu[0] = 0.05
+    f0 = lbm.equilibrium(torch.ones((nx, ny, nz), dtype=torch.float32, device=dev), uu)
+    _, _, F = A.rollout(f0, 20, tau=0.56, q=q, porosity=hard, collision="mrt")
+    cd = A.drag_series(F, 0.05, torch.clamp(lbm.projected_area(solid), min=1.0))[-1]
+    assert float(cd) > 0.0
+    (g,) = torch.autograd.grad(cd, r)
+    assert float(g) == 0.0
+
+
+@pytest.mark.parametrize("policy", [3, (7, 3)])
+def test_checkpointed_gradient_equals_store_all(dev, policy):
+    nx, ny, nz = 24, 12, 12
+    solid, q = lbm.create_sphere(nx, ny, nz, (-1.0, 0.0, 0.0), 0.6, device=dev)
+    f0 = lbm.equilibrium(torch.ones((nx, ny, nz), device=dev), torch.zeros((3, nx, ny, nz), device=dev))
+    out = []
+    for ck in ("all", policy):
+        qq = q.clone().requires_grad_(True)
+        _, _, F = A.rollout(f0, 23, tau=0.55, q=qq, solid=solid, collision="mrt", guard="pre", checkpoint=ck)
+        (g,) = torch.autograd.grad(F[:, 0].sum(), qq)
+        out.append(cpu(g))
+    assert torch.equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("storage,tol", [("f32", 1e-4), ("f16", 2e-2)])
+def test_reduced_precision_gradient_close_to_f64(dev, storage, tol):
+    nx, ny, nz = 32, 16, 16
+    res = {}
+    for st in ("f64", storage):
+        dt = torch.float64 if st == "f64" else torch.float32
+        r = torch.tensor(0.5, dtype=dt, device=dev, requires_grad=True)
+        por, solid, q = lbm.smooth_sphere_geometry(nx, ny, nz, (-1.0, 0.0, 0.0), r, 20.0, out_dtype=dt, device=dev)
+        hard = solid.to(dt)
+        uu = torch.zeros((3, nx, ny, nz), dtype=dt, device=dev)
+        f0 = lbm.equilibrium(torch.ones((nx, ny, nz), dtype=dt, device=dev), uu)
+        _, _, F = A.rollout(f0, 60, tau=0.55, q=q, porosity=hard, collision="mrt", storage=st, offset="inlet")
+        cd = A.drag_series(F, 0.05, torch.clamp(lbm.projected_area(solid).to(torch.float64), min=1.0))[-20:].mean()
+        (g,) = torch.autograd.grad(cd, r)
+        res[st] = (float(cd), float(g))
+    assert abs(res[storage][1] - res["f64"][1]) <= tol * abs(res["f64"][1])
