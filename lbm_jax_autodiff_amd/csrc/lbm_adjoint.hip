@@ -21,7 +21,7 @@ constexpr int kAdjBlock = 256;
 
 template <typename R>
 struct AdjArgs {
-  // forward configuration (same meaning as StepArgs)
+  // forward configuration (same meaning as in the forward kernel)
   int nx, ny, nz;
   uint32_t N;
   FastDiv dnx, dny;
@@ -51,27 +51,30 @@ __device__ __forceinline__ void adj_coords(const AdjArgs<R>& a, uint32_t c, int&
   z = int(zz);
 }
 
-// cell x - e_i with all three axes wrapped: the entry whose normal consumer is x
-template <typename R>
-__device__ __forceinline__ uint32_t owner_target(const AdjArgs<R>& a, int i, int x, int y, int z) {
-  int sx = x - EX[i], sy = y - EY[i], sz = z - EZ[i];
+// cell x - e_I with all three axes wrapped: the entry whose normal consumer is x
+template <int I, typename R>
+__device__ __forceinline__ uint32_t owner_target(const AdjArgs<R>& a, int x, int y, int z) {
+  int sx = x - EX[I], sy = y - EY[I], sz = z - EZ[I];
   sx = sx < 0 ? sx + a.nx : (sx >= a.nx ? sx - a.nx : sx);
   sy = sy < 0 ? sy + a.ny : (sy >= a.ny ? sy - a.ny : sy);
   sz = sz < 0 ? sz + a.nz : (sz >= a.nz ? sz - a.nz : sz);
   return (uint32_t(sz) * uint32_t(a.ny) + uint32_t(sy)) * uint32_t(a.nx) + uint32_t(sx);
 }
 
-template <typename R>
-__device__ __forceinline__ uint32_t next_cell(const AdjArgs<R>& a, int i, int x, int y, int z) {
-  int fx = x + EX[i], fy = y + EY[i], fz = z + EZ[i];
+template <int I, typename R>
+__device__ __forceinline__ uint32_t next_cell(const AdjArgs<R>& a, int x, int y, int z) {
+  int fx = x + EX[I], fy = y + EY[I], fz = z + EZ[I];
   fx = fx < 0 ? fx + a.nx : (fx >= a.nx ? fx - a.nx : fx);
   fy = fy < 0 ? fy + a.ny : (fy >= a.ny ? fy - a.ny : fy);
   fz = fz < 0 ? fz + a.nz : (fz >= a.nz ? fz - a.nz : fz);
   return (uint32_t(fz) * uint32_t(a.ny) + uint32_t(fy)) * uint32_t(a.nx) + uint32_t(fx);
 }
 
-__device__ __forceinline__ bool x_fallback(int i, int x, int nx) {
-  return (EX[i] == 1 && x == 0) || (EX[i] == -1 && x == nx - 1);
+template <int I>
+__device__ __forceinline__ bool x_fallback(int x, int nx) {
+  if constexpr (EX[I] == 1) return x == 0;
+  else if constexpr (EX[I] == -1) return x == nx - 1;
+  else return false;
 }
 
 // Pulled values of cell x (normal pulls and, for link cells, Bouzidi).
@@ -81,41 +84,84 @@ __device__ __forceinline__ void pull_cell(const AdjArgs<typename S::R>& a, const
                                           typename S::R* own) {
   using R = typename S::R;
   const size_t N = a.N;
-  if (LINK) {
-#pragma unroll
-    for (int i = 0; i < Q; ++i) own[i] = S::ld_c(P, i * N + c, a.off[i]);
+  if constexpr (LINK) {
+    sfor<Q>([&](auto I) {
+      constexpr int i = I;
+      own[i] = S::ld_c(P, i * N + c, a.off[i]);
+    });
     s[0] = own[0];
   } else {
     s[0] = S::ld_c(P, c, a.off[0]);
   }
-#pragma unroll
-  for (int i = 1; i < Q; ++i) {
-    bool link = false;
-    R qo = R(-1);
-    if (LINK) {
-      qo = R(a.link_q[size_t(opp(i) - 1) * a.n_links + b]);
-      link = qo >= R(0);
-    }
-    if (link) {
-      const int io = opp(i);
-      if (qo >= R(0.5)) {
-        const R inv2q = R(1) / (R(2) * fmin(fmax(qo, R(0.5)), R(1)));
-        s[i] = inv2q * own[io] + (R(1) - inv2q) * own[i];
-      } else {
-        const R twoq = R(2) * fmin(fmax(qo, R(0)), R(0.5));
-        s[i] = twoq * own[io] + (R(1) - twoq) * S::ld_c(P, io * N + next_cell(a, i, x, y, z), a.off[io]);
+  sfor<Q>([&](auto I) {
+    constexpr int i = I;
+    if constexpr (i != 0) {
+      constexpr int io = opp(i);
+      bool link = false;
+      R qo = R(-1);
+      if constexpr (LINK) {
+        qo = R(a.link_q[size_t(io - 1) * a.n_links + b]);
+        link = qo >= R(0);
       }
-    } else {
-      const uint32_t src = x_fallback(i, x, a.nx) ? c : owner_target(a, i, x, y, z);
-      s[i] = S::ld_c(P, i * N + src, a.off[i]);
+      if (link) {
+        if constexpr (LINK) {
+          if (qo >= R(0.5)) {
+            const R inv2q = R(1) / (R(2) * fmin(fmax(qo, R(0.5)), R(1)));
+            s[i] = inv2q * own[io] + (R(1) - inv2q) * own[i];
+          } else {
+            const R twoq = R(2) * fmin(fmax(qo, R(0)), R(0.5));
+            s[i] = twoq * own[io] + (R(1) - twoq) * S::ld_c(P, io * N + next_cell<i>(a, x, y, z), a.off[io]);
+          }
+        }
+      } else {
+        const uint32_t src = x_fallback<i>(x, a.nx) ? c : owner_target<i>(a, x, y, z);
+        s[i] = S::ld_c(P, i * N + src, a.off[i]);
+      }
     }
-  }
+  });
 }
 
 template <typename R>
 __device__ __forceinline__ void apply_zou_he(const AdjArgs<R>& a, R* s, int x) {
   if (x == 0) zou_he_in(s, a.uin[0], a.uin[1], a.uin[2], false, R(0));
   if (x == a.nx - 1) zou_he_out(s, a.rho_out);
+}
+
+// g <- (pre-guard, collide, bounce)^T g at the collision input s (the cell's
+// S_{t+1}); s is replaced by the pre-guard reset state where the guard fires.
+// Accumulates dL/dporosity.  Returns true when the pre guard fired.
+template <typename R, int COLL>
+__device__ __forceinline__ bool collide_transpose(const AdjArgs<R>& a, R* s, R* g, uint32_t c) {
+  bool pre_reset = false;
+  if (a.guard == LBM_GUARD_PRE && guard_bad(s)) {
+    pre_reset = true;
+    set_feq_unit(s, a.uref[0], a.uref[1], a.uref[2]);
+  }
+  const R tau_e = a.smag ? smag_tau(s, a.tau, a.cs) : a.tau;
+  const R omega = R(1) / tau_e;
+  if (a.bounce == LBM_BOUNCE_HARD) {
+    if (a.flags[c] & 1) reverse_all(g);
+  } else if (a.bounce == LBM_BOUNCE_SOFT) {
+    R cc[Q];
+    sfor<Q>([&](auto I) {
+      constexpr int i = I;
+      cc[i] = s[i];
+    });
+    if constexpr (COLL == LBM_COLL_MRT) collide_mrt(cc, omega);
+    else collide_bgk(cc, omega);
+    const R gp = blend_vjp(cc, g, a.phi[c]);
+    if (a.phi_acc) a.phi_acc[c] += double(gp);
+  }
+  if (pre_reset) {
+    sfor<Q>([&](auto I) {
+      constexpr int i = I;
+      g[i] = R(0);
+    });
+  } else {
+    const R gom = (COLL == LBM_COLL_MRT) ? mrt_vjp(s, g, omega) : bgk_vjp(s, g, omega);
+    if (a.smag) smag_vjp(s, g, a.tau, a.cs, -gom * omega * omega);
+  }
+  return pre_reset;
 }
 
 // One cell: forward recompute + cotangent pull-back + owner writes.
@@ -133,126 +179,81 @@ __device__ __forceinline__ void adjoint_cell(const AdjArgs<typename S::R>& a, co
   pull_cell<S, LINK>(a, P, c, x, y, z, b, s, own);
   apply_zou_he(a, s, x);
   const bool post_reset = (a.guard == LBM_GUARD_POST) && guard_bad(s);
-  if (post_reset) {
-#pragma unroll
-    for (int i = 0; i < Q; ++i) s[i] = R(weight(i));
-  }
+  if (post_reset) set_weights(s);
   R g[Q];
-  if (MODE == LBM_MODE_FULL) {
-    bool pre_reset = false;
-    if (a.guard == LBM_GUARD_PRE && guard_bad(s)) {
-      pre_reset = true;
-      const R uu = a.uref[0] * a.uref[0] + a.uref[1] * a.uref[1] + a.uref[2] * a.uref[2];
-#pragma unroll
-      for (int i = 0; i < Q; ++i) s[i] = feq(i, R(1), a.uref[0], a.uref[1], a.uref[2], uu);
-    }
-#pragma unroll
-    for (int i = 0; i < Q; ++i) g[i] = lam_in[i * N + c];
-    // bounce^T
-    if (a.bounce == LBM_BOUNCE_HARD && (a.flags[c] & 1)) {
-#pragma unroll
-      for (int i = 1; i < Q; ++i) {
-        if (opp(i) > i) {
-          const R t = g[i];
-          g[i] = g[opp(i)];
-          g[opp(i)] = t;
-        }
-      }
-    } else if (a.bounce == LBM_BOUNCE_SOFT) {
-      const R p = a.phi[c];
-      R cc[Q];
-      const R tau_e = a.smag ? smag_tau(s, a.tau, a.cs) : a.tau;
-#pragma unroll
-      for (int i = 0; i < Q; ++i) cc[i] = s[i];
-      if (COLL == LBM_COLL_MRT) collide_mrt(cc, R(1) / tau_e);
-      else collide_bgk(cc, R(1) / tau_e);
-      R gphi = R(0);
-#pragma unroll
-      for (int i = 0; i < Q; ++i) gphi += g[i] * (cc[opp(i)] - cc[i]);
-      if (a.phi_acc) a.phi_acc[c] += double(gphi);
-      R t[Q];
-#pragma unroll
-      for (int i = 0; i < Q; ++i) t[i] = g[i];
-#pragma unroll
-      for (int i = 0; i < Q; ++i) g[i] = (R(1) - p) * t[i] + p * t[opp(i)];
-    }
-    // collision^T (and the Smagorinsky tau_eff path)
-    if (pre_reset) {
-#pragma unroll
-      for (int i = 0; i < Q; ++i) g[i] = R(0);
-    } else {
-      const R tau_e = a.smag ? smag_tau(s, a.tau, a.cs) : a.tau;
-      const R omega = R(1) / tau_e;
-      const R gom = (COLL == LBM_COLL_MRT) ? mrt_vjp(s, g, omega) : bgk_vjp(s, g, omega);
-      if (a.smag) smag_vjp(s, g, a.tau, a.cs, -gom * omega * omega);
-    }
-  } else {
-#pragma unroll
-    for (int i = 0; i < Q; ++i) g[i] = lam_in[i * N + c];
-  }
+  sfor<Q>([&](auto I) {
+    constexpr int i = I;
+    g[i] = lam_in[i * N + c];
+  });
+  if constexpr (MODE == LBM_MODE_FULL) collide_transpose<R, COLL>(a, s, g, c);
   // momentum-exchange force uses S_{t+1}(x)_opp(j) for links j (forces.py:95-118)
-  if (LINK && gF) {
-    const R gx = R(gF[0]), gy = R(gF[1]), gz = R(gF[2]);
-#pragma unroll
-    for (int j = 1; j < Q; ++j) {
-      if (a.link_q[size_t(j - 1) * a.n_links + b] >= 0.0) {
-        R v = R(0);
-        if (EX[j]) v += EX[j] > 0 ? gx : -gx;
-        if (EY[j]) v += EY[j] > 0 ? gy : -gy;
-        if (EZ[j]) v += EZ[j] > 0 ? gz : -gz;
-        g[opp(j)] += v;
-      }
+  if constexpr (LINK) {
+    if (gF) {
+      const R gx = R(gF[0]), gy = R(gF[1]), gz = R(gF[2]);
+      sfor<Q>([&](auto J) {
+        constexpr int j = J;
+        if constexpr (j != 0) {
+          if (a.link_q[size_t(j - 1) * a.n_links + b] >= 0.0) {
+            R v = R(0);
+            add_scaled<EX[j]>(v, gx);
+            add_scaled<EY[j]>(v, gy);
+            add_scaled<EZ[j]>(v, gz);
+            g[opp(j)] += v;
+          }
+        }
+      });
     }
   }
   if (post_reset) {
-#pragma unroll
-    for (int i = 0; i < Q; ++i) g[i] = R(0);
+    sfor<Q>([&](auto I) {
+      constexpr int i = I;
+      g[i] = R(0);
+    });
   }
-  // Zou-He^T (outlet was applied last)
-  if (x == 0 || x == a.nx - 1) {
+  // Zou-He^T (the outlet was applied last); the inlet needs the pulled values
+  if (x == a.nx - 1) zou_he_out_vjp(g, a.rho_out);
+  if (x == 0) {
     R sp[Q], tmp[LINK ? Q : 1];
     pull_cell<S, LINK>(a, P, c, x, y, z, b, sp, tmp);
-    if (x == a.nx - 1) zou_he_out_vjp(g, a.rho_out);
-    if (x == 0) {
-      double gu[3] = {0.0, 0.0, 0.0};
-      zou_he_in_vjp(sp, g, a.uin[0], a.uin[1], a.uin[2], gu);
-      if (a.u_acc) {
-        const size_t row = size_t(c) / size_t(a.nx);
-        a.u_acc[3 * row] += gu[0];
-        a.u_acc[3 * row + 1] += gu[1];
-        a.u_acc[3 * row + 2] += gu[2];
-      }
+    double gu[3] = {0.0, 0.0, 0.0};
+    zou_he_in_vjp(sp, g, a.uin[0], a.uin[1], a.uin[2], gu);
+    if (a.u_acc) {
+      const size_t row = size_t(c) / size_t(a.nx);
+      a.u_acc[3 * row] += gu[0];
+      a.u_acc[3 * row + 1] += gu[1];
+      a.u_acc[3 * row + 2] += gu[2];
     }
   }
   // owner writes and Bouzidi bookkeeping
   lam_out[c] = g[0];
-#pragma unroll
-  for (int i = 1; i < Q; ++i) {
-    bool link = false;
-    R qo = R(-1);
-    if (LINK) {
-      qo = R(a.link_q[size_t(opp(i) - 1) * a.n_links + b]);
-      link = qo >= R(0);
-    }
-    const uint32_t tgt = owner_target(a, i, x, y, z);
-    const bool normal = !link && !x_fallback(i, x, a.nx);
-    lam_out[i * N + tgt] = normal ? g[i] : R(0);
-    if (LINK) {
-      a.mu_link[size_t(i) * a.n_links + b] = link ? g[i] : R(0);
-      if (link && a.dq_acc) {
-        // d S_i / d q for the taken branch; 0 outside clip bounds
-        const int io = opp(i);
-        R dsdq = R(0);
-        if (qo >= R(0.5)) {
-          if (qo <= R(1)) dsdq = -(own[io] - own[i]) / (R(2) * qo * qo);
-        } else {
-          const R cval = S::ld_c(P, io * N + next_cell(a, i, x, y, z), a.off[io]);
-          dsdq = R(2) * (own[io] - cval);
+  sfor<Q>([&](auto I) {
+    constexpr int i = I;
+    if constexpr (i != 0) {
+      constexpr int io = opp(i);
+      bool link = false;
+      R qo = R(-1);
+      if constexpr (LINK) {
+        qo = R(a.link_q[size_t(io - 1) * a.n_links + b]);
+        link = qo >= R(0);
+      }
+      const bool normal = !link && !x_fallback<i>(x, a.nx);
+      lam_out[i * N + owner_target<i>(a, x, y, z)] = normal ? g[i] : R(0);
+      if constexpr (LINK) {
+        a.mu_link[size_t(i) * a.n_links + b] = link ? g[i] : R(0);
+        if (link && a.dq_acc) {
+          // d S_i / d q on the taken branch; 0 outside the clip bounds
+          R dsdq = R(0);
+          if (qo >= R(0.5)) {
+            if (qo <= R(1)) dsdq = -(own[io] - own[i]) / (R(2) * qo * qo);
+          } else {
+            const R cval = S::ld_c(P, io * N + next_cell<i>(a, x, y, z), a.off[io]);
+            dsdq = R(2) * (own[io] - cval);
+          }
+          a.dq_acc[size_t(io - 1) * a.n_links + b] += double(g[i]) * double(dsdq);
         }
-        a.dq_acc[size_t(io - 1) * a.n_links + b] += double(g[i]) * double(dsdq);
       }
     }
-  }
+  });
 }
 
 template <class S, int COLL, int MODE>
@@ -285,56 +286,16 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjoint_collide(AdjArgs<typename 
   if (c >= a.N) return;
   const size_t N = a.N;
   R s[Q], g[Q];
-#pragma unroll
-  for (int i = 0; i < Q; ++i) s[i] = S::ld_c(S0, i * N + c, a.off[i]);
-  bool pre_reset = false;
-  if (a.guard == LBM_GUARD_PRE && guard_bad(s)) {
-    pre_reset = true;
-    const R uu = a.uref[0] * a.uref[0] + a.uref[1] * a.uref[1] + a.uref[2] * a.uref[2];
-#pragma unroll
-    for (int i = 0; i < Q; ++i) s[i] = feq(i, R(1), a.uref[0], a.uref[1], a.uref[2], uu);
-  }
-#pragma unroll
-  for (int i = 0; i < Q; ++i) g[i] = lam_in[i * N + c];
-  if (a.bounce == LBM_BOUNCE_HARD && (a.flags[c] & 1)) {
-#pragma unroll
-    for (int i = 1; i < Q; ++i) {
-      if (opp(i) > i) {
-        const R t = g[i];
-        g[i] = g[opp(i)];
-        g[opp(i)] = t;
-      }
-    }
-  } else if (a.bounce == LBM_BOUNCE_SOFT) {
-    const R p = a.phi[c];
-    R cc[Q];
-    const R tau_e = a.smag ? smag_tau(s, a.tau, a.cs) : a.tau;
-#pragma unroll
-    for (int i = 0; i < Q; ++i) cc[i] = s[i];
-    if (COLL == LBM_COLL_MRT) collide_mrt(cc, R(1) / tau_e);
-    else collide_bgk(cc, R(1) / tau_e);
-    R gphi = R(0);
-#pragma unroll
-    for (int i = 0; i < Q; ++i) gphi += g[i] * (cc[opp(i)] - cc[i]);
-    if (a.phi_acc) a.phi_acc[c] += double(gphi);
-    R t[Q];
-#pragma unroll
-    for (int i = 0; i < Q; ++i) t[i] = g[i];
-#pragma unroll
-    for (int i = 0; i < Q; ++i) g[i] = (R(1) - p) * t[i] + p * t[opp(i)];
-  }
-  if (pre_reset) {
-    // the reset state does not depend on S_0
-#pragma unroll
-    for (int i = 0; i < Q; ++i) g[i] = R(0);
-  } else {
-    const R tau_e = a.smag ? smag_tau(s, a.tau, a.cs) : a.tau;
-    const R omega = R(1) / tau_e;
-    const R gom = (COLL == LBM_COLL_MRT) ? mrt_vjp(s, g, omega) : bgk_vjp(s, g, omega);
-    if (a.smag) smag_vjp(s, g, a.tau, a.cs, -gom * omega * omega);
-  }
-#pragma unroll
-  for (int i = 0; i < Q; ++i) g_out[i * N + c] = g[i];
+  sfor<Q>([&](auto I) {
+    constexpr int i = I;
+    s[i] = S::ld_c(S0, i * N + c, a.off[i]);
+    g[i] = lam_in[i * N + c];
+  });
+  collide_transpose<R, COLL>(a, s, g, c);
+  sfor<Q>([&](auto I) {
+    constexpr int i = I;
+    g_out[i * N + c] = g[i];
+  });
 }
 
 // Deterministic additions: lam[tgt[k]] += sum_c coef[c] * mu_link[src[c]]
@@ -353,16 +314,10 @@ __global__ void k_adjoint_scatter(int n_targets, const int32_t* __restrict__ tgt
     } else if (gF) {
       const int j = -1 - s;
       R v = R(0);
-      const R gx = R(gF[0]), gy = R(gF[1]), gz = R(gF[2]);
-      switch (j) {  // e_j . gF
-        default: break;
-#define LBM_FORCE_CASE(J) case J: v = R(EX[J]) * gx + R(EY[J]) * gy + R(EZ[J]) * gz; break;
-        LBM_FORCE_CASE(1) LBM_FORCE_CASE(2) LBM_FORCE_CASE(3) LBM_FORCE_CASE(4) LBM_FORCE_CASE(5)
-        LBM_FORCE_CASE(6) LBM_FORCE_CASE(7) LBM_FORCE_CASE(8) LBM_FORCE_CASE(9) LBM_FORCE_CASE(10)
-        LBM_FORCE_CASE(11) LBM_FORCE_CASE(12) LBM_FORCE_CASE(13) LBM_FORCE_CASE(14) LBM_FORCE_CASE(15)
-        LBM_FORCE_CASE(16) LBM_FORCE_CASE(17) LBM_FORCE_CASE(18)
-#undef LBM_FORCE_CASE
-      }
+      sfor<Q>([&](auto J) {
+        constexpr int jj = J;
+        if (j == jj) v = R(EX[jj]) * R(gF[0]) + R(EY[jj]) * R(gF[1]) + R(EZ[jj]) * R(gF[2]);
+      });
       acc += v;
     }
   }

@@ -12,49 +12,46 @@
 
 namespace lbm {
 
-// Pull back (G_rho|u, G_u) -- the cotangents of rho and u at fixed other
-// variable -- to the populations through rho = sum f, u = j / rho:
-//   g_i += G_rho - (G_u . u)/rho + (G_u . e_i)/rho.
+// Pull back (G_rho|u, G_u) -- the cotangents of rho and u taken as
+// independent variables -- to the populations through rho = sum f,
+// u = j / rho:   g_i += G_rho - (G_u . u)/rho + (G_u . e_i)/rho.
 template <typename R>
 __device__ __forceinline__ void add_macro_pullback(R* g, const Macro<R>& m, R Grho, R Gux, R Guy, R Guz) {
   const R inv = R(1) / m.rho;
   const R a = Grho - (Gux * m.ux + Guy * m.uy + Guz * m.uz) * inv;
   const R bx = Gux * inv, by = Guy * inv, bz = Guz * inv;
-#pragma unroll
-  for (int i = 0; i < Q; ++i) {
+  sfor<Q>([&](auto I) {
+    constexpr int i = I;
     R v = a;
-    if (EX[i]) v += EX[i] > 0 ? bx : -bx;
-    if (EY[i]) v += EY[i] > 0 ? by : -by;
-    if (EZ[i]) v += EZ[i] > 0 ? bz : -bz;
+    add_scaled<EX[i]>(v, bx);
+    add_scaled<EY[i]>(v, by);
+    add_scaled<EZ[i]>(v, bz);
     g[i] += v;
-  }
+  });
 }
 
 // Cotangents of (rho, u) from a cotangent h on f_eq(rho, u):
-//   d feq_k / d rho|u = feq_k / rho,
+//   d feq_k / d rho|u = w_k (1 + 3 e.u + 4.5 (e.u)^2 - 1.5 u^2),
 //   d feq_k / d u_a   = w_k rho (3 e_ka + 9 (e_k.u) e_ka - 3 u_a).
 template <typename R>
 __device__ __forceinline__ void feq_vjp(const R* h, const Macro<R>& m, R& Grho, R& Gux, R& Guy, R& Guz) {
-  Grho = 0;
-  Gux = 0;
-  Guy = 0;
-  Guz = 0;
-  R hw = 0;  // sum_k h_k w_k
-#pragma unroll
-  for (int k = 0; k < Q; ++k) {
-    R eu = R(0);
-    if (EX[k]) eu += EX[k] > 0 ? m.ux : -m.ux;
-    if (EY[k]) eu += EY[k] > 0 ? m.uy : -m.uy;
-    if (EZ[k]) eu += EZ[k] > 0 ? m.uz : -m.uz;
-    const R wk = R(weight(k));
-    const R hk = h[k] * wk;
+  Grho = R(0);
+  Gux = R(0);
+  Guy = R(0);
+  Guz = R(0);
+  R hw = R(0);  // sum_k h_k w_k
+  sfor<Q>([&](auto K) {
+    constexpr int k = K;
+    constexpr float wk = weight(k);
+    const R eu = edot<k>(m.ux, m.uy, m.uz);
+    const R hk = h[k] * R(wk);
     Grho += hk * (R(1) + R(3) * eu + R(4.5) * eu * eu - R(1.5) * m.usq);
     hw += hk;
     const R t = hk * (R(3) + R(9) * eu);
-    if (EX[k]) Gux += EX[k] > 0 ? t : -t;
-    if (EY[k]) Guy += EY[k] > 0 ? t : -t;
-    if (EZ[k]) Guz += EZ[k] > 0 ? t : -t;
-  }
+    add_scaled<EX[k]>(Gux, t);
+    add_scaled<EY[k]>(Guy, t);
+    add_scaled<EZ[k]>(Guz, t);
+  });
   Gux = m.rho * (Gux - R(3) * m.ux * hw);
   Guy = m.rho * (Guy - R(3) * m.uy * hw);
   Guz = m.rho * (Guz - R(3) * m.uz * hw);
@@ -65,34 +62,34 @@ template <typename R>
 __device__ __forceinline__ R bgk_vjp(const R* s, R* g, R omega) {
   const Macro<R> m = macro(s);
   R h[Q];
-  R gom = 0;
-#pragma unroll
-  for (int i = 0; i < Q; ++i) {
-    const R e = feq(i, m.rho, m.ux, m.uy, m.uz, m.usq);
+  R gom = R(0);
+  sfor<Q>([&](auto I) {
+    constexpr int i = I;
+    const R e = feq<i>(m.rho, m.ux, m.uy, m.uz, m.usq);
     gom -= g[i] * (s[i] - e);
     h[i] = omega * g[i];
     g[i] = (R(1) - omega) * g[i];
-  }
+  });
   R Gr, Gx, Gy, Gz;
   feq_vjp(h, m, Gr, Gx, Gy, Gz);
   add_macro_pullback(g, m, Gr, Gx, Gy, Gz);
   return gom;
 }
 
-// d meq_k / d rho|u and d meq_k / d u (non-conserved rows), contracted with z.
+// Contraction of z with d meq / d(rho|u) and d meq / d u over the
+// non-conserved rows (lattice.py:369-391).
 template <typename R>
 __device__ __forceinline__ void meq_vjp(const R* z, const Macro<R>& m, R& Grho, R& Gux, R& Guy, R& Guz) {
   const R r = m.rho, ux = m.ux, uy = m.uy, uz = m.uz, u2 = m.usq;
   const R dxx = R(2) * ux * ux - uy * uy - uz * uz;
   const R dww = uy * uy - uz * uz;
   const R c23 = R(-2.0 / 3.0);
-  Grho = z[1] * (R(-11) + R(19) * u2) + z[2] * (R(3) - R(5.5) * u2) + c23 * (z[4] * ux + z[6] * uy + z[8] * uz) +
-         (z[9] - R(0.5) * z[10]) * dxx + (z[11] - R(0.5) * z[12]) * dww + z[13] * ux * uy + z[14] * uy * uz +
-         z[15] * ux * uz;
   // rows 9/10 and 11/12 share their u-dependence up to the factor -1/2
   const R a = z[9] - R(0.5) * z[10];
   const R b = z[11] - R(0.5) * z[12];
   const R e = R(38) * z[1] - R(11) * z[2];
+  Grho = z[1] * (R(-11) + R(19) * u2) + z[2] * (R(3) - R(5.5) * u2) + c23 * (z[4] * ux + z[6] * uy + z[8] * uz) +
+         a * dxx + b * dww + z[13] * ux * uy + z[14] * uy * uz + z[15] * ux * uz;
   Gux = r * (e * ux + c23 * z[4] + R(4) * a * ux + z[13] * uy + z[15] * uz);
   Guy = r * (e * uy + c23 * z[6] - R(2) * a * uy + R(2) * b * uy + z[13] * ux + z[14] * uz);
   Guz = r * (e * uz + c23 * z[8] - R(2) * a * uz - R(2) * b * uz + z[14] * uy + z[15] * ux);
@@ -104,36 +101,24 @@ template <typename R>
 __device__ __forceinline__ R mrt_vjp(const R* s, R* g, R omega) {
   const Macro<R> mac = macro(s);
   R h[Q];
-  R gom = 0;
-#pragma unroll
-  for (int k = 0; k < Q; ++k) {
-    if (mrt_is_conserved(k)) {
+  R gom = R(0);
+  sfor<Q>([&](auto K) {
+    constexpr int k = K;
+    if constexpr (mrt_is_conserved(k)) {
       h[k] = R(0);
-      continue;
-    }
-    const R Mg = moment(k, g);
-    const R inv_n = R(1.0 / mrt_norm(k));
-    if (mrt_is_stress(k)) {
-      const R dm = moment(k, s) - meq(k, mac.rho, mac.ux, mac.uy, mac.uz, mac.usq);
-      gom -= Mg * dm * inv_n;
-      h[k] = omega * Mg * inv_n;
     } else {
-      h[k] = R(mrt_rate_fixed(k)) * Mg * inv_n;
+      constexpr double inv_n = 1.0 / double(mrt_norm(k));
+      const R Mg = moment<k>(g);
+      if constexpr (mrt_is_stress(k)) {
+        const R dm = moment<k>(s) - meq<k>(mac.rho, mac.ux, mac.uy, mac.uz, mac.usq);
+        gom -= Mg * dm * R(inv_n);
+        h[k] = omega * Mg * R(inv_n);
+      } else {
+        h[k] = R(mrt_rate_fixed(k)) * Mg * R(inv_n);
+      }
     }
-  }
-#pragma unroll
-  for (int i = 0; i < Q; ++i) {
-    R acc = R(0);
-#pragma unroll
-    for (int k = 0; k < Q; ++k) {
-      if (mrt_is_conserved(k)) continue;
-      const int c = mrt_m(k, i);
-      if (c == 1) acc += h[k];
-      else if (c == -1) acc -= h[k];
-      else if (c != 0) acc += R(c) * h[k];
-    }
-    g[i] -= acc;
-  }
+  });
+  sub_transposed(g, h);
   R Gr, Gx, Gy, Gz;
   meq_vjp(h, mac, Gr, Gx, Gy, Gz);
   add_macro_pullback(g, mac, Gr, Gx, Gy, Gz);
@@ -141,51 +126,41 @@ __device__ __forceinline__ R mrt_vjp(const R* s, R* g, R omega) {
 }
 
 // Smagorinsky: tau_eff(s) = (tau + sqrt(tau^2 + 18 cs^2 |Pi| / (rho + 1e-10))) / 2,
-// Pi_ab = sum_i e_ia e_ib (s_i - feq_i(s)).  Adds g_tau_eff * d tau_eff / d s to g.
+// Pi_ab = sum_i e_ia e_ib (s_i - feq_i(s)).  Adds g_teff * d tau_eff / d s to g.
 template <typename R>
 __device__ __forceinline__ void smag_vjp(const R* s, R* g, R tau, R cs, R g_teff) {
   const Macro<R> m = macro(s);
-  R sxx = 0, syy = 0, szz = 0, sxy = 0, sxz = 0, syz = 0;
-#pragma unroll
-  for (int i = 0; i < Q; ++i) {
-    const R n = s[i] - feq(i, m.rho, m.ux, m.uy, m.uz, m.usq);
-    if (EX[i]) sxx += n;
-    if (EY[i]) syy += n;
-    if (EZ[i]) szz += n;
-    if (EX[i] * EY[i]) sxy += EX[i] * EY[i] > 0 ? n : -n;
-    if (EX[i] * EZ[i]) sxz += EX[i] * EZ[i] > 0 ? n : -n;
-    if (EY[i] * EZ[i]) syz += EY[i] * EZ[i] > 0 ? n : -n;
-  }
-  const R pi = sqrt(sxx * sxx + syy * syy + szz * szz + R(2) * (sxy * sxy + sxz * sxz + syz * syz));
+  const Stress<R> st = neq_stress(s, m);
+  const R pi = sqrt(st.xx * st.xx + st.yy * st.yy + st.zz * st.zz +
+                    R(2) * (st.xy * st.xy + st.xz * st.xz + st.yz * st.yz));
   const R den = m.rho + R(1e-10);
   const R k18 = R(18) * cs * cs;
   const R root = sqrt(tau * tau + k18 * pi / den);
-  // d tau_eff / d pi and d tau_eff / d rho (explicit)
+  // d tau_eff / d pi and the explicit d tau_eff / d rho
   const R dpi = R(0.25) * k18 / (den * root);
   const R drho = R(-0.25) * k18 * pi / (den * den * root);
   const R gpi = g_teff * dpi;
   // subgradient 0 at |Pi| = 0
   const R inv = pi > R(0) ? gpi / pi : R(0);
-  const R gxx = inv * sxx, gyy = inv * syy, gzz = inv * szz;
-  const R gxy = R(2) * inv * sxy, gxz = R(2) * inv * sxz, gyz = R(2) * inv * syz;
-  // cotangent on the non-equilibrium part n_i = s_i - feq_i
+  const R gxx = inv * st.xx, gyy = inv * st.yy, gzz = inv * st.zz;
+  const R gxy = R(2) * inv * st.xy, gxz = R(2) * inv * st.xz, gyz = R(2) * inv * st.yz;
+  // cotangent of the non-equilibrium part n_i = s_i - feq_i
   R hn[Q];
-#pragma unroll
-  for (int i = 0; i < Q; ++i) {
+  sfor<Q>([&](auto I) {
+    constexpr int i = I;
     R v = R(0);
-    if (EX[i]) v += gxx;
-    if (EY[i]) v += gyy;
-    if (EZ[i]) v += gzz;
-    if (EX[i] * EY[i]) v += EX[i] * EY[i] > 0 ? gxy : -gxy;
-    if (EX[i] * EZ[i]) v += EX[i] * EZ[i] > 0 ? gxz : -gxz;
-    if (EY[i] * EZ[i]) v += EY[i] * EZ[i] > 0 ? gyz : -gyz;
+    add_scaled<EX[i] * EX[i]>(v, gxx);
+    add_scaled<EY[i] * EY[i]>(v, gyy);
+    add_scaled<EZ[i] * EZ[i]>(v, gzz);
+    add_scaled<EX[i] * EY[i]>(v, gxy);
+    add_scaled<EX[i] * EZ[i]>(v, gxz);
+    add_scaled<EY[i] * EZ[i]>(v, gyz);
     hn[i] = v;
     g[i] += v;
-  }
+  });
   R Gr, Gx, Gy, Gz;
   feq_vjp(hn, m, Gr, Gx, Gy, Gz);
-  const R grho_direct = g_teff * drho;
-  add_macro_pullback(g, m, grho_direct - Gr, -Gx, -Gy, -Gz);
+  add_macro_pullback(g, m, g_teff * drho - Gr, -Gx, -Gy, -Gz);
 }
 
 // Zou-He inlet transpose.  s holds the pulled (pre-inlet) values; g the
@@ -228,6 +203,20 @@ __device__ __forceinline__ void zou_he_out_vjp(R* g, R rho_out) {
   g[9] += R(2) * a + g8;
   g[11] += R(2) * a + g14;
   g[13] += R(2) * a + g12;
+}
+
+// Soft-bounce transpose: returns dL/dp = sum_i g_i (c_opp(i) - c_i) and
+// overwrites g with (1 - p) g + p g[opp].
+template <typename R>
+__device__ __forceinline__ R blend_vjp(const R* c, R* g, R p) {
+  R gp = R(0);
+  sfor<Q>([&](auto I) {
+    constexpr int i = I;
+    constexpr int o = opp(i);
+    gp += g[i] * (c[o] - c[i]);
+  });
+  blend_opposite(g, p);
+  return gp;
 }
 
 }  // namespace lbm

@@ -10,11 +10,28 @@
 #include <hip/hip_bf16.h>
 #include <stdint.h>
 
+#include <type_traits>
+#include <utility>
+
 #include "../../include/lbm_hip.h"
 
 namespace lbm {
 
 constexpr int Q = 19;
+
+// Compile-time loop: sfor<N>(f) calls f(std::integral_constant<int, 0>{}) ...
+// f(std::integral_constant<int, N-1>{}).  Inside f, `constexpr int i = I;`
+// makes every lattice table lookup a constant expression, so the moment
+// matrix, weights and opposite map fold into immediates with no run-time
+// table code.
+template <typename F, int... Is>
+__host__ __device__ __forceinline__ void sfor_impl(F&& f, std::integer_sequence<int, Is...>) {
+  (f(std::integral_constant<int, Is>{}), ...);
+}
+template <int N, typename F>
+__host__ __device__ __forceinline__ void sfor(F&& f) {
+  sfor_impl(f, std::make_integer_sequence<int, N>{});
+}
 
 // D3Q19 velocity set in the order of jax_lbm/core/lattice.py:29-49: rest, the
 // six faces (+x,-x,+y,-y,+z,-z), then the xy, xz and yz edges.

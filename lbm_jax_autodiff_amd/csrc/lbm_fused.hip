@@ -44,29 +44,61 @@ __device__ __forceinline__ void cell_coords(const StepArgs<R>& a, uint32_t c, in
   z = int(zz);
 }
 
-// Source cell of the normal pull for direction i (streaming.py:113-142);
-// returns c itself where the reference keeps the cell's own value.
+// Neighbour rows of a cell, wrapped (periodic) or flagged (clamped).
+struct Rows {
+  uint32_t ym, yp, zm, zp;  // row offsets (in cells) of y-1, y+1, z-1, z+1
+  bool ym_ok, yp_ok, zm_ok, zp_ok;
+};
+
 template <typename R>
-__device__ __forceinline__ uint32_t pull_src(const StepArgs<R>& a, int i, uint32_t c, int x, int y, int z) {
-  const int ex = EX[i], ey = EY[i], ez = EZ[i];
-  if (ex == 1 && x == 0) return c;
-  if (ex == -1 && x == a.nx - 1) return c;
-  int sy = y - ey, sz = z - ez;
-  if (a.periodic) {
-    if (sy < 0) sy += a.ny;
-    if (sy >= a.ny) sy -= a.ny;
-    if (sz < 0) sz += a.nz;
-    if (sz >= a.nz) sz -= a.nz;
-  } else {
-    if (sy < 0 || sy >= a.ny || sz < 0 || sz >= a.nz) return c;
-  }
-  return (uint32_t(sz) * uint32_t(a.ny) + uint32_t(sy)) * uint32_t(a.nx) + uint32_t(x - ex);
+__device__ __forceinline__ Rows make_rows(const StepArgs<R>& a, int y, int z) {
+  Rows r;
+  const uint32_t nx = uint32_t(a.nx), plane = uint32_t(a.nx) * uint32_t(a.ny);
+  r.ym = (y == 0) ? (uint32_t(a.ny - 1) * nx) : uint32_t(0) - nx;
+  r.yp = (y == a.ny - 1) ? uint32_t(0) - uint32_t(a.ny - 1) * nx : nx;
+  r.zm = (z == 0) ? (uint32_t(a.nz - 1) * plane) : uint32_t(0) - plane;
+  r.zp = (z == a.nz - 1) ? uint32_t(0) - uint32_t(a.nz - 1) * plane : plane;
+  r.ym_ok = a.periodic || y > 0;
+  r.yp_ok = a.periodic || y < a.ny - 1;
+  r.zm_ok = a.periodic || z > 0;
+  r.zp_ok = a.periodic || z < a.nz - 1;
+  return r;
 }
 
-// Cell x + e_i wrapped periodically in all three axes (streaming.py:166-172).
-template <typename R>
-__device__ __forceinline__ uint32_t wrap_next(const StepArgs<R>& a, int i, int x, int y, int z) {
-  int fx = x + EX[i], fy = y + EY[i], fz = z + EZ[i];
+// Source cell of the normal pull for direction I (streaming.py:113-142):
+// x - e_I with y/z wrapped; the cell itself where the reference keeps the
+// cell's own value (x faces, and y/z faces when not periodic).  Offsets are
+// added modulo 2^32, so a wrapped row is a plain unsigned addition.
+template <int I, typename R>
+__device__ __forceinline__ uint32_t pull_src(const StepArgs<R>& a, const Rows& r, uint32_t c, int x) {
+  constexpr int ex = EX[I], ey = EY[I], ez = EZ[I];
+  if constexpr (ex == 1) {
+    if (x == 0) return c;
+  } else if constexpr (ex == -1) {
+    if (x == a.nx - 1) return c;
+  }
+  uint32_t s = c - uint32_t(ex);
+  if constexpr (ey == 1) {
+    if (!r.ym_ok) return c;
+    s += r.ym;
+  } else if constexpr (ey == -1) {
+    if (!r.yp_ok) return c;
+    s += r.yp;
+  }
+  if constexpr (ez == 1) {
+    if (!r.zm_ok) return c;
+    s += r.zm;
+  } else if constexpr (ez == -1) {
+    if (!r.zp_ok) return c;
+    s += r.zp;
+  }
+  return s;
+}
+
+// Cell x + e_I wrapped periodically in all three axes (streaming.py:166-172).
+template <int I, typename R>
+__device__ __forceinline__ uint32_t wrap_next(const StepArgs<R>& a, int x, int y, int z) {
+  int fx = x + EX[I], fy = y + EY[I], fz = z + EZ[I];
   fx = fx < 0 ? fx + a.nx : (fx >= a.nx ? fx - a.nx : fx);
   fy = fy < 0 ? fy + a.ny : (fy >= a.ny ? fy - a.ny : fy);
   fz = fz < 0 ? fz + a.nz : (fz >= a.nz ? fz - a.nz : fz);
@@ -78,45 +110,34 @@ template <typename R>
 __device__ __forceinline__ void boundaries(const StepArgs<R>& a, R* s, int x) {
   if (x == 0) zou_he_in(s, a.uin[0], a.uin[1], a.uin[2], false, R(0));
   if (x == a.nx - 1) zou_he_out(s, a.rho_out);
-  if (a.guard == LBM_GUARD_POST && guard_bad(s)) {
-#pragma unroll
-    for (int i = 0; i < Q; ++i) s[i] = R(weight(i));
-  }
+  if (a.guard == LBM_GUARD_POST && guard_bad(s)) set_weights(s);
 }
 
 // [pre guard] -> [Smagorinsky tau] -> BGK|MRT -> hard|soft bounce
 // (examples/10:71-78, state.py:133-149, smooth.py:151-155).
 template <typename R, int COLL>
 __device__ __forceinline__ void collide_cell(const StepArgs<R>& a, R* s, uint32_t c) {
-  if (a.guard == LBM_GUARD_PRE && guard_bad(s)) {
-    const R uu = a.uref[0] * a.uref[0] + a.uref[1] * a.uref[1] + a.uref[2] * a.uref[2];
-#pragma unroll
-    for (int i = 0; i < Q; ++i) s[i] = feq(i, R(1), a.uref[0], a.uref[1], a.uref[2], uu);
-  }
+  if (a.guard == LBM_GUARD_PRE && guard_bad(s)) set_feq_unit(s, a.uref[0], a.uref[1], a.uref[2]);
   R tau = a.tau;
   if (a.smag) tau = smag_tau(s, a.tau, a.cs);
   const R omega = R(1) / tau;
-  if (COLL == LBM_COLL_MRT) collide_mrt(s, omega);
+  if constexpr (COLL == LBM_COLL_MRT) collide_mrt(s, omega);
   else collide_bgk(s, omega);
   if (a.bounce == LBM_BOUNCE_HARD) {
-    if (a.flags[c] & 1) {
-#pragma unroll
-      for (int i = 1; i < Q; ++i) {
-        if (opp(i) > i) {
-          const R t = s[i];
-          s[i] = s[opp(i)];
-          s[opp(i)] = t;
-        }
-      }
-    }
+    if (a.flags[c] & 1) reverse_all(s);
   } else if (a.bounce == LBM_BOUNCE_SOFT) {
-    const R p = a.phi[c];
-    R t[Q];
-#pragma unroll
-    for (int i = 0; i < Q; ++i) t[i] = s[i];
-#pragma unroll
-    for (int i = 0; i < Q; ++i) s[i] = (R(1) - p) * t[i] + p * t[opp(i)];
+    blend_opposite(s, a.phi[c]);
   }
+}
+
+template <class S>
+__device__ __forceinline__ void store_cell(const StepArgs<typename S::R>& a, typename S::T* __restrict__ dst,
+                                           uint32_t c, const typename S::R* s) {
+  const size_t N = a.N;
+  sfor<Q>([&](auto I) {
+    constexpr int i = I;
+    S::st(dst, i * N + c, s[i], a.off[i]);
+  });
 }
 
 template <class S, int COLL, int MODE>
@@ -136,41 +157,47 @@ __global__ __launch_bounds__(kStepBlock) void k_step(StepArgs<typename S::R> a,
       const uint32_t c = uint32_t(a.link_cell[b]);
       int x, y, z;
       cell_coords(a, c, x, y, z);
+      const Rows rows = make_rows(a, y, z);
       R own[Q], s[Q];
-#pragma unroll
-      for (int i = 0; i < Q; ++i) own[i] = S::ld_c(src, i * N + c, a.off[i]);
+      sfor<Q>([&](auto I) {
+        constexpr int i = I;
+        own[i] = S::ld_c(src, i * N + c, a.off[i]);
+      });
       s[0] = own[0];
-#pragma unroll
-      for (int i = 1; i < Q; ++i) {
-        const int io = opp(i);
-        const R qo = R(a.link_q[size_t(io - 1) * a.n_links + b]);
-        if (qo >= R(0)) {
-          if (qo >= R(0.5)) {
-            const R inv2q = R(1) / (R(2) * fmin(fmax(qo, R(0.5)), R(1)));
-            s[i] = inv2q * own[io] + (R(1) - inv2q) * own[i];
+      sfor<Q>([&](auto I) {
+        constexpr int i = I;
+        if constexpr (i != 0) {
+          constexpr int io = opp(i);
+          const R qo = R(a.link_q[size_t(io - 1) * a.n_links + b]);
+          if (qo >= R(0)) {
+            if (qo >= R(0.5)) {
+              const R inv2q = R(1) / (R(2) * fmin(fmax(qo, R(0.5)), R(1)));
+              s[i] = inv2q * own[io] + (R(1) - inv2q) * own[i];
+            } else {
+              const R twoq = R(2) * fmin(fmax(qo, R(0)), R(0.5));
+              const R ffv = S::ld_c(src, io * N + wrap_next<i>(a, x, y, z), a.off[io]);
+              s[i] = twoq * own[io] + (R(1) - twoq) * ffv;
+            }
           } else {
-            const R twoq = R(2) * fmin(fmax(qo, R(0)), R(0.5));
-            const R ffv = S::ld_c(src, io * N + wrap_next(a, i, x, y, z), a.off[io]);
-            s[i] = twoq * own[io] + (R(1) - twoq) * ffv;
+            s[i] = S::ld_c(src, i * N + pull_src<i>(a, rows, c, x), a.off[i]);
           }
-        } else {
-          s[i] = S::ld_c(src, i * N + pull_src(a, i, c, x, y, z), a.off[i]);
         }
-      }
+      });
       boundaries(a, s, x);
       // momentum exchange: links j with q_j(x) >= 0 (forces.py:95-118)
-#pragma unroll
-      for (int j = 1; j < Q; ++j) {
-        if (a.link_q[size_t(j - 1) * a.n_links + b] >= 0.0) {
-          const double t = double(own[j]) + double(s[opp(j)]);
-          if (EX[j]) fx += EX[j] > 0 ? t : -t;
-          if (EY[j]) fy += EY[j] > 0 ? t : -t;
-          if (EZ[j]) fz += EZ[j] > 0 ? t : -t;
+      sfor<Q>([&](auto J) {
+        constexpr int j = J;
+        if constexpr (j != 0) {
+          if (a.link_q[size_t(j - 1) * a.n_links + b] >= 0.0) {
+            const double t = double(own[j]) + double(s[opp(j)]);
+            add_scaled<EX[j]>(fx, t);
+            add_scaled<EY[j]>(fy, t);
+            add_scaled<EZ[j]>(fz, t);
+          }
         }
-      }
+      });
       if (MODE == LBM_MODE_FULL) collide_cell<R, COLL>(a, s, c);
-#pragma unroll
-      for (int i = 0; i < Q; ++i) S::st(dst, i * N + c, s[i], a.off[i]);
+      store_cell<S>(a, dst, c, s);
     }
     if (fpart) {
       __shared__ double red[3][kStepBlock / 64];
@@ -201,20 +228,24 @@ __global__ __launch_bounds__(kStepBlock) void k_step(StepArgs<typename S::R> a,
   if (c >= a.N) return;
   R s[Q];
   if (MODE == LBM_MODE_COLLIDE) {
-#pragma unroll
-    for (int i = 0; i < Q; ++i) s[i] = S::ld(src, i * N + c, a.off[i]);
+    sfor<Q>([&](auto I) {
+      constexpr int i = I;
+      s[i] = S::ld(src, i * N + c, a.off[i]);
+    });
     collide_cell<R, COLL>(a, s, c);
   } else {
     if (a.flags && (a.flags[c] & 2)) return;  // owned by a link workgroup
     int x, y, z;
     cell_coords(a, c, x, y, z);
-#pragma unroll
-    for (int i = 0; i < Q; ++i) s[i] = S::ld_c(src, i * N + pull_src(a, i, c, x, y, z), a.off[i]);
+    const Rows rows = make_rows(a, y, z);
+    sfor<Q>([&](auto I) {
+      constexpr int i = I;
+      s[i] = S::ld_c(src, i * N + pull_src<i>(a, rows, c, x), a.off[i]);
+    });
     boundaries(a, s, x);
     if (MODE == LBM_MODE_FULL) collide_cell<R, COLL>(a, s, c);
   }
-#pragma unroll
-  for (int i = 0; i < Q; ++i) S::st(dst, i * N + c, s[i], a.off[i]);
+  store_cell<S>(a, dst, c, s);
 }
 
 __global__ void k_force_reduce(const double* __restrict__ part, int T, int slots, double* __restrict__ force) {

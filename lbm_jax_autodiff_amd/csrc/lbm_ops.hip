@@ -26,8 +26,26 @@ __global__ void k_equilibrium(const R* __restrict__ rho, const R* __restrict__ u
   if (c >= N) return;
   const R r = rho[c], ux = u[c], uy = u[N + c], uz = u[2 * N + c];
   const R usq = ux * ux + uy * uy + uz * uz;
-#pragma unroll
-  for (int i = 0; i < Q; ++i) f[i * N + c] = feq(i, r, ux, uy, uz, usq);
+  sfor<Q>([&](auto I) {
+    constexpr int i = I;
+    f[i * N + c] = feq<i>(r, ux, uy, uz, usq);
+  });
+}
+
+template <typename R>
+__device__ __forceinline__ void load_cell(const R* __restrict__ f, size_t N, size_t c, R* v) {
+  sfor<Q>([&](auto I) {
+    constexpr int i = I;
+    v[i] = f[i * N + c];
+  });
+}
+
+template <typename R>
+__device__ __forceinline__ void store_cell(R* __restrict__ f, size_t N, size_t c, const R* v) {
+  sfor<Q>([&](auto I) {
+    constexpr int i = I;
+    f[i * N + c] = v[i];
+  });
 }
 
 template <typename R>
@@ -35,8 +53,10 @@ __global__ void k_density(const R* __restrict__ f, R* __restrict__ rho, size_t N
   const size_t c = blockIdx.x * size_t(blockDim.x) + threadIdx.x;
   if (c >= N) return;
   R acc = R(0);
-#pragma unroll
-  for (int i = 0; i < Q; ++i) acc += f[i * N + c];
+  sfor<Q>([&](auto I) {
+    constexpr int i = I;
+    acc += f[i * N + c];
+  });
   rho[c] = acc;
 }
 
@@ -46,13 +66,13 @@ __global__ void k_velocity(const R* __restrict__ f, const R* __restrict__ rho, R
   const size_t c = blockIdx.x * size_t(blockDim.x) + threadIdx.x;
   if (c >= N) return;
   R jx = 0, jy = 0, jz = 0;
-#pragma unroll
-  for (int i = 0; i < Q; ++i) {
+  sfor<Q>([&](auto I) {
+    constexpr int i = I;
     const R v = f[i * N + c];
-    if (EX[i]) jx += EX[i] > 0 ? v : -v;
-    if (EY[i]) jy += EY[i] > 0 ? v : -v;
-    if (EZ[i]) jz += EZ[i] > 0 ? v : -v;
-  }
+    add_scaled<EX[i]>(jx, v);
+    add_scaled<EY[i]>(jy, v);
+    add_scaled<EZ[i]>(jz, v);
+  });
   const R r = rho[c];
   u[c] = jx / r;
   u[N + c] = jy / r;
@@ -65,13 +85,11 @@ __global__ void k_collide(const R* __restrict__ fin, R tau, const R* __restrict_
   const size_t c = blockIdx.x * size_t(blockDim.x) + threadIdx.x;
   if (c >= N) return;
   R f[Q];
-#pragma unroll
-  for (int i = 0; i < Q; ++i) f[i] = fin[i * N + c];
+  load_cell(fin, N, c, f);
   const R omega = R(1) / (tau_field ? tau_field[c] : tau);
-  if (MRT) collide_mrt(f, omega);
+  if constexpr (MRT) collide_mrt(f, omega);
   else collide_bgk(f, omega);
-#pragma unroll
-  for (int i = 0; i < Q; ++i) fout[i * N + c] = f[i];
+  store_cell(fout, N, c, f);
 }
 
 template <typename R>
@@ -79,8 +97,7 @@ __global__ void k_smag(const R* __restrict__ fin, R tau, R cs, R* __restrict__ t
   const size_t c = blockIdx.x * size_t(blockDim.x) + threadIdx.x;
   if (c >= N) return;
   R f[Q];
-#pragma unroll
-  for (int i = 0; i < Q; ++i) f[i] = fin[i * N + c];
+  load_cell(fin, N, c, f);
   tau_out[c] = smag_tau(f, tau, cs);
 }
 
@@ -90,11 +107,9 @@ __global__ void k_soft_bounce(const R* __restrict__ fin, const R* __restrict__ p
   const size_t c = blockIdx.x * size_t(blockDim.x) + threadIdx.x;
   if (c >= N) return;
   R f[Q];
-#pragma unroll
-  for (int i = 0; i < Q; ++i) f[i] = fin[i * N + c];
-  const R p = phi[c];
-#pragma unroll
-  for (int i = 0; i < Q; ++i) fout[i * N + c] = (R(1) - p) * f[i] + p * f[opp(i)];
+  load_cell(fin, N, c, f);
+  blend_opposite(f, phi[c]);
+  store_cell(fout, N, c, f);
 }
 
 template <typename R>
@@ -102,9 +117,10 @@ __global__ void k_hard_bounce(const R* __restrict__ fin, const int32_t* __restri
                               R* __restrict__ fout, size_t N) {
   const size_t c = blockIdx.x * size_t(blockDim.x) + threadIdx.x;
   if (c >= N) return;
-  const bool s = solid[c] >= 1;
-#pragma unroll
-  for (int i = 0; i < Q; ++i) fout[i * N + c] = fin[(s ? opp(i) : i) * N + c];
+  R f[Q];
+  load_cell(fin, N, c, f);
+  if (solid[c] >= 1) reverse_all(f);
+  store_cell(fout, N, c, f);
 }
 
 // Pull streaming + Bouzidi (streaming.py:99-183), dense q.
@@ -118,34 +134,36 @@ __global__ void k_stream(const R* __restrict__ f, const R* __restrict__ q, int p
   const int y = int((c / nx) % ny);
   const int z = int(c / (size_t(nx) * ny));
   out[c] = f[c];
-#pragma unroll
-  for (int i = 1; i < Q; ++i) {
-    const int io = opp(i);
-    // normal pull from x - e_i
-    int sx = x - EX[i], sy = y - EY[i], sz = z - EZ[i];
-    bool own = (sx < 0) || (sx >= nx);
-    if (periodic) {
-      sy = (sy + ny) % ny;
-      sz = (sz + nz) % nz;
-    } else {
-      own = own || sy < 0 || sy >= ny || sz < 0 || sz >= nz;
-    }
-    R v = own ? f[i * N + c] : f[i * N + (size_t(sz) * ny + sy) * nx + sx];
-    const R qo = q[io * N + c];
-    if (qo >= R(0)) {
-      const R a = f[io * N + c];
-      if (qo >= R(0.5)) {
-        const R inv2q = R(1) / (R(2) * fmin(fmax(qo, R(0.5)), R(1)));
-        v = inv2q * a + (R(1) - inv2q) * f[i * N + c];
+  sfor<Q>([&](auto I) {
+    constexpr int i = I;
+    if constexpr (i != 0) {
+      constexpr int io = opp(i);
+      // normal pull from x - e_i
+      int sx = x - EX[i], sy = y - EY[i], sz = z - EZ[i];
+      bool own = (sx < 0) || (sx >= nx);
+      if (periodic) {
+        sy = (sy + ny) % ny;
+        sz = (sz + nz) % nz;
       } else {
-        // x + e_i, periodic in all three axes (streaming.py:166-172)
-        const int fx = (x + EX[i] + nx) % nx, fy = (y + EY[i] + ny) % ny, fz = (z + EZ[i] + nz) % nz;
-        const R twoq = R(2) * fmin(fmax(qo, R(0)), R(0.5));
-        v = twoq * a + (R(1) - twoq) * f[io * N + (size_t(fz) * ny + fy) * nx + fx];
+        own = own || sy < 0 || sy >= ny || sz < 0 || sz >= nz;
       }
+      R v = own ? f[i * N + c] : f[i * N + (size_t(sz) * ny + sy) * nx + sx];
+      const R qo = q[io * N + c];
+      if (qo >= R(0)) {
+        const R a = f[io * N + c];
+        if (qo >= R(0.5)) {
+          const R inv2q = R(1) / (R(2) * fmin(fmax(qo, R(0.5)), R(1)));
+          v = inv2q * a + (R(1) - inv2q) * f[i * N + c];
+        } else {
+          // x + e_i, periodic in all three axes (streaming.py:166-172)
+          const int fx = (x + EX[i] + nx) % nx, fy = (y + EY[i] + ny) % ny, fz = (z + EZ[i] + nz) % nz;
+          const R twoq = R(2) * fmin(fmax(qo, R(0)), R(0.5));
+          v = twoq * a + (R(1) - twoq) * f[io * N + (size_t(fz) * ny + fy) * nx + fx];
+        }
+      }
+      out[i * N + c] = v;
     }
-    out[i * N + c] = v;
-  }
+  });
 }
 
 template <typename R>
@@ -156,8 +174,7 @@ __global__ void k_zou_he_in(R* __restrict__ f, R ux, R uy, R uz, int has_rt, R r
   if (p >= size_t(ny) * nz) return;
   const size_t c = p * nx;  // x = 0 of row p
   R g[Q];
-#pragma unroll
-  for (int i = 0; i < Q; ++i) g[i] = f[i * N + c];
+  load_cell(f, N, c, g);
   zou_he_in(g, ux, uy, uz, has_rt != 0, rt);
   f[1 * N + c] = g[1];
   f[7 * N + c] = g[7];
@@ -173,8 +190,7 @@ __global__ void k_zou_he_out(R* __restrict__ f, R rho_out, int nx, int ny, int n
   if (p >= size_t(ny) * nz) return;
   const size_t c = p * nx + (nx - 1);
   R g[Q];
-#pragma unroll
-  for (int i = 0; i < Q; ++i) g[i] = f[i * N + c];
+  load_cell(f, N, c, g);
   zou_he_out(g, rho_out);
   f[2 * N + c] = g[2];
   f[10 * N + c] = g[10];
@@ -194,27 +210,31 @@ __global__ void k_mex_partial(const R* __restrict__ fp, const R* __restrict__ fs
     R eqv[Q];
     if (want_p) {
       R f[Q];
-#pragma unroll
-      for (int i = 0; i < Q; ++i) f[i] = fp[i * N + c];
+      load_cell(fp, N, c, f);
       const Macro<R> m = macro(f);
-#pragma unroll
-      for (int i = 0; i < Q; ++i) eqv[i] = feq(i, m.rho, m.ux, m.uy, m.uz, m.usq);
+      sfor<Q>([&](auto I) {
+        constexpr int i = I;
+        eqv[i] = feq<i>(m.rho, m.ux, m.uy, m.uz, m.usq);
+      });
     }
-#pragma unroll
-    for (int i = 1; i < Q; ++i) {
-      if (q[i * N + c] >= R(0)) {
-        const double t = double(fp[i * N + c]) + double(fs[opp(i) * N + c]);
-        acc[0] += EX[i] * t;
-        acc[1] += EY[i] * t;
-        acc[2] += EZ[i] * t;
-        if (want_p) {
-          const double pp = double(eqv[i]) + double(eqv[opp(i)]);
-          acc[3] += EX[i] * pp;
-          acc[4] += EY[i] * pp;
-          acc[5] += EZ[i] * pp;
+    sfor<Q>([&](auto I) {
+      constexpr int i = I;
+      if constexpr (i != 0) {
+        constexpr int io = opp(i);
+        if (q[i * N + c] >= R(0)) {
+          const double t = double(fp[i * N + c]) + double(fs[io * N + c]);
+          add_scaled<EX[i]>(acc[0], t);
+          add_scaled<EY[i]>(acc[1], t);
+          add_scaled<EZ[i]>(acc[2], t);
+          if (want_p) {
+            const double pp = double(eqv[i]) + double(eqv[io]);
+            add_scaled<EX[i]>(acc[3], pp);
+            add_scaled<EY[i]>(acc[4], pp);
+            add_scaled<EZ[i]>(acc[5], pp);
+          }
         }
       }
-    }
+    });
   }
   __shared__ double sh[6][kBlock / 64];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;

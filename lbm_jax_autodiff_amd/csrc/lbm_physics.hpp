@@ -1,6 +1,8 @@
 // Cell-local D3Q19 physics shared by the per-op and the fused kernels.
-// Everything here works on 19 values held in registers; loops over the
-// velocity set are fully unrolled so that lattice constants fold away.
+// Everything works on 19 values held in registers.  Loops over the velocity
+// set and the moment rows use sfor, so every lattice constant (velocity
+// components, weights, opposites, integer moment-matrix entries) is a
+// compile-time constant and zero terms are never emitted.
 #pragma once
 
 #include "lbm_common.hpp"
@@ -12,17 +14,26 @@ struct Macro {
   R rho, ux, uy, uz, usq;
 };
 
+// acc += c * v for a compile-time integer c, without multiplications by +-1
+// and with nothing emitted for c == 0.
+template <int C, typename R>
+__device__ __forceinline__ void add_scaled(R& acc, R v) {
+  if constexpr (C == 1) acc += v;
+  else if constexpr (C == -1) acc -= v;
+  else if constexpr (C != 0) acc += R(C) * v;
+}
+
 // rho = sum f_i; u = sum f_i e_i / rho  (equilibrium.py:22-87)
 template <typename R>
 __device__ __forceinline__ Macro<R> macro(const R* f) {
   R rho = R(0), jx = R(0), jy = R(0), jz = R(0);
-#pragma unroll
-  for (int i = 0; i < Q; ++i) {
+  sfor<Q>([&](auto I) {
+    constexpr int i = I;
     rho += f[i];
-    if (EX[i]) jx += EX[i] > 0 ? f[i] : -f[i];
-    if (EY[i]) jy += EY[i] > 0 ? f[i] : -f[i];
-    if (EZ[i]) jz += EZ[i] > 0 ? f[i] : -f[i];
-  }
+    add_scaled<EX[i]>(jx, f[i]);
+    add_scaled<EY[i]>(jy, f[i]);
+    add_scaled<EZ[i]>(jz, f[i]);
+  });
   Macro<R> m;
   m.rho = rho;
   m.ux = jx / rho;
@@ -32,60 +43,77 @@ __device__ __forceinline__ Macro<R> macro(const R* f) {
   return m;
 }
 
+// e_i . u for a compile-time direction
+template <int I, typename R>
+__device__ __forceinline__ R edot(R ux, R uy, R uz) {
+  R eu = R(0);
+  add_scaled<EX[I]>(eu, ux);
+  add_scaled<EY[I]>(eu, uy);
+  add_scaled<EZ[I]>(eu, uz);
+  return eu;
+}
+
 // f_eq_i = w_i rho (1 + e.u/cs2 + (e.u)^2/(2 cs2^2) - u.u/(2 cs2))
 // (equilibrium.py:132-151) with cs2 = 1/3.
-template <typename R>
-__device__ __forceinline__ R feq(int i, R rho, R ux, R uy, R uz, R usq) {
-  R eu = R(0);
-  if (EX[i]) eu += EX[i] > 0 ? ux : -ux;
-  if (EY[i]) eu += EY[i] > 0 ? uy : -uy;
-  if (EZ[i]) eu += EZ[i] > 0 ? uz : -uz;
-  return R(weight(i)) * rho * (R(1) + R(3) * eu + R(4.5) * eu * eu - R(1.5) * usq);
+template <int I, typename R>
+__device__ __forceinline__ R feq(R rho, R ux, R uy, R uz, R usq) {
+  constexpr float w = weight(I);
+  const R eu = edot<I>(ux, uy, uz);
+  return R(w) * rho * (R(1) + R(3) * eu + R(4.5) * eu * eu - R(1.5) * usq);
 }
 
 // BGK relaxation (collision.py:89-100)
 template <typename R>
 __device__ __forceinline__ void collide_bgk(R* f, R omega) {
   const Macro<R> m = macro(f);
-#pragma unroll
-  for (int i = 0; i < Q; ++i) {
-    const R e = feq(i, m.rho, m.ux, m.uy, m.uz, m.usq);
+  sfor<Q>([&](auto I) {
+    constexpr int i = I;
+    const R e = feq<i>(m.rho, m.ux, m.uy, m.uz, m.usq);
     f[i] = f[i] - omega * (f[i] - e);
-  }
+  });
 }
 
 // Equilibrium moments (lattice.py:342-391) for the non-conserved rows.
-template <typename R>
-__device__ __forceinline__ R meq(int k, R rho, R ux, R uy, R uz, R usq) {
-  switch (k) {
-    case 1: return R(-11) * rho + R(19) * rho * usq;
-    case 2: return R(3) * rho - R(5.5) * rho * usq;
-    case 4: return R(-2.0 / 3.0) * rho * ux;
-    case 6: return R(-2.0 / 3.0) * rho * uy;
-    case 8: return R(-2.0 / 3.0) * rho * uz;
-    case 9: return rho * (R(2) * ux * ux - uy * uy - uz * uz);
-    case 10: return R(-0.5) * rho * (R(2) * ux * ux - uy * uy - uz * uz);
-    case 11: return rho * (uy * uy - uz * uz);
-    case 12: return R(-0.5) * rho * (uy * uy - uz * uz);
-    case 13: return rho * ux * uy;
-    case 14: return rho * uy * uz;
-    case 15: return rho * ux * uz;
-    default: return R(0);
-  }
+template <int K, typename R>
+__device__ __forceinline__ R meq(R rho, R ux, R uy, R uz, R usq) {
+  if constexpr (K == 1) return R(-11) * rho + R(19) * rho * usq;
+  else if constexpr (K == 2) return R(3) * rho - R(5.5) * rho * usq;
+  else if constexpr (K == 4) return R(-2.0 / 3.0) * rho * ux;
+  else if constexpr (K == 6) return R(-2.0 / 3.0) * rho * uy;
+  else if constexpr (K == 8) return R(-2.0 / 3.0) * rho * uz;
+  else if constexpr (K == 9) return rho * (R(2) * ux * ux - uy * uy - uz * uz);
+  else if constexpr (K == 10) return R(-0.5) * rho * (R(2) * ux * ux - uy * uy - uz * uz);
+  else if constexpr (K == 11) return rho * (uy * uy - uz * uz);
+  else if constexpr (K == 12) return R(-0.5) * rho * (uy * uy - uz * uz);
+  else if constexpr (K == 13) return rho * ux * uy;
+  else if constexpr (K == 14) return rho * uy * uz;
+  else if constexpr (K == 15) return rho * ux * uz;
+  else return R(0);
 }
 
-// Moment k of f: sum_i M_ki f_i (integer coefficients fold at compile time).
-template <typename R>
-__device__ __forceinline__ R moment(int k, const R* f) {
+// Moment K of f: sum_i M_Ki f_i (integer coefficients folded at compile time).
+template <int K, typename R>
+__device__ __forceinline__ R moment(const R* f) {
   R acc = R(0);
-#pragma unroll
-  for (int i = 0; i < Q; ++i) {
-    const int c = mrt_m(k, i);
-    if (c == 1) acc += f[i];
-    else if (c == -1) acc -= f[i];
-    else if (c != 0) acc += R(c) * f[i];
-  }
+  sfor<Q>([&](auto I) {
+    constexpr int i = I;
+    add_scaled<mrt_m(K, i)>(acc, f[i]);
+  });
   return acc;
+}
+
+// acc_i -= sum_k M_ki d_k over the non-conserved rows (the transposed transform)
+template <typename R>
+__device__ __forceinline__ void sub_transposed(R* f, const R* d) {
+  sfor<Q>([&](auto I) {
+    constexpr int i = I;
+    R acc = R(0);
+    sfor<Q>([&](auto K) {
+      constexpr int k = K;
+      if constexpr (!mrt_is_conserved(k)) add_scaled<mrt_m(k, i)>(acc, d[k]);
+    });
+    f[i] -= acc;
+  });
 }
 
 // MRT relaxation (collision.py:149-184).  Written as
@@ -96,49 +124,50 @@ template <typename R>
 __device__ __forceinline__ void collide_mrt(R* f, R omega) {
   const Macro<R> mac = macro(f);
   R d[Q];
-#pragma unroll
-  for (int k = 0; k < Q; ++k) {
-    if (mrt_is_conserved(k)) {
+  sfor<Q>([&](auto K) {
+    constexpr int k = K;
+    if constexpr (mrt_is_conserved(k)) {
       d[k] = R(0);
-      continue;
+    } else {
+      constexpr double inv_n = 1.0 / double(mrt_norm(k));
+      const R rate = mrt_is_stress(k) ? omega : R(mrt_rate_fixed(k));
+      d[k] = rate * R(inv_n) * (moment<k>(f) - meq<k>(mac.rho, mac.ux, mac.uy, mac.uz, mac.usq));
     }
-    const R rate = mrt_is_stress(k) ? omega : R(mrt_rate_fixed(k));
-    const R mk = moment(k, f);
-    d[k] = rate * (mk - meq(k, mac.rho, mac.ux, mac.uy, mac.uz, mac.usq)) * R(1.0 / mrt_norm(k));
-  }
-#pragma unroll
-  for (int i = 0; i < Q; ++i) {
-    R acc = R(0);
-#pragma unroll
-    for (int k = 0; k < Q; ++k) {
-      if (mrt_is_conserved(k)) continue;
-      const int c = mrt_m(k, i);
-      if (c == 1) acc += d[k];
-      else if (c == -1) acc -= d[k];
-      else if (c != 0) acc += R(c) * d[k];
-    }
-    f[i] -= acc;
-  }
+  });
+  sub_transposed(f, d);
 }
 
-// Non-equilibrium stress Pi_ab = sum_i e_ia e_ib (f_i - f_eq_i) and
+// Non-equilibrium stress Pi_ab = sum_i e_ia e_ib (f_i - f_eq_i).
+template <typename R>
+struct Stress {
+  R xx, yy, zz, xy, xz, yz;
+};
+
+template <typename R>
+__device__ __forceinline__ Stress<R> neq_stress(const R* f, const Macro<R>& m) {
+  Stress<R> s{R(0), R(0), R(0), R(0), R(0), R(0)};
+  sfor<Q>([&](auto I) {
+    constexpr int i = I;
+    if constexpr (i != 0) {
+      const R n = f[i] - feq<i>(m.rho, m.ux, m.uy, m.uz, m.usq);
+      add_scaled<EX[i] * EX[i]>(s.xx, n);
+      add_scaled<EY[i] * EY[i]>(s.yy, n);
+      add_scaled<EZ[i] * EZ[i]>(s.zz, n);
+      add_scaled<EX[i] * EY[i]>(s.xy, n);
+      add_scaled<EX[i] * EZ[i]>(s.xz, n);
+      add_scaled<EY[i] * EZ[i]>(s.yz, n);
+    }
+  });
+  return s;
+}
+
 // tau_eff = (tau + sqrt(tau^2 + 18 cs^2 |Pi| / (rho + 1e-10))) / 2
 // (collision.py:241-270; |Pi| is the Frobenius norm of the full tensor).
 template <typename R>
 __device__ __forceinline__ R smag_tau(const R* f, R tau, R cs) {
   const Macro<R> m = macro(f);
-  R sxx = 0, syy = 0, szz = 0, sxy = 0, sxz = 0, syz = 0;
-#pragma unroll
-  for (int i = 0; i < Q; ++i) {
-    const R n = f[i] - feq(i, m.rho, m.ux, m.uy, m.uz, m.usq);
-    if (EX[i]) sxx += n;
-    if (EY[i]) syy += n;
-    if (EZ[i]) szz += n;
-    if (EX[i] * EY[i]) sxy += EX[i] * EY[i] > 0 ? n : -n;
-    if (EX[i] * EZ[i]) sxz += EX[i] * EZ[i] > 0 ? n : -n;
-    if (EY[i] * EZ[i]) syz += EY[i] * EZ[i] > 0 ? n : -n;
-  }
-  const R pi = sqrt(sxx * sxx + syy * syy + szz * szz + R(2) * (sxy * sxy + sxz * sxz + syz * syz));
+  const Stress<R> s = neq_stress(f, m);
+  const R pi = sqrt(s.xx * s.xx + s.yy * s.yy + s.zz * s.zz + R(2) * (s.xy * s.xy + s.xz * s.xz + s.yz * s.yz));
   return R(0.5) * (tau + sqrt(tau * tau + R(18) * cs * cs * pi / (m.rho + R(1e-10))));
 }
 
@@ -175,6 +204,55 @@ __device__ __forceinline__ bool guard_bad(const R* f) {
   const Macro<R> m = macro(f);
   const R speed = sqrt(m.usq);
   return (m.rho < R(0.3)) | (m.rho > R(2.0)) | (speed > R(0.4)) | (m.rho != m.rho);
+}
+
+// f <- f[opp] (hard solid bounce, state.py:146-149)
+template <typename R>
+__device__ __forceinline__ void reverse_all(R* f) {
+  sfor<Q>([&](auto I) {
+    constexpr int i = I;
+    constexpr int o = opp(i);
+    if constexpr (o > i) {
+      const R t = f[i];
+      f[i] = f[o];
+      f[o] = t;
+    }
+  });
+}
+
+// f <- (1 - p) f + p f[opp] (soft bounce, smooth.py:151-155)
+template <typename R>
+__device__ __forceinline__ void blend_opposite(R* f, R p) {
+  sfor<Q>([&](auto I) {
+    constexpr int i = I;
+    constexpr int o = opp(i);
+    if constexpr (o > i) {
+      const R a = f[i], b = f[o];
+      f[i] = (R(1) - p) * a + p * b;
+      f[o] = (R(1) - p) * b + p * a;
+    } else if constexpr (o == i) {
+      // rest population: (1 - p) f0 + p f0
+      f[i] = (R(1) - p) * f[i] + p * f[i];
+    }
+  });
+}
+
+// f_i = f_eq_i(1, u) (the guard reset targets)
+template <typename R>
+__device__ __forceinline__ void set_feq_unit(R* f, R ux, R uy, R uz) {
+  const R uu = ux * ux + uy * uy + uz * uz;
+  sfor<Q>([&](auto I) {
+    constexpr int i = I;
+    f[i] = feq<i>(R(1), ux, uy, uz, uu);
+  });
+}
+
+template <typename R>
+__device__ __forceinline__ void set_weights(R* f) {
+  sfor<Q>([&](auto I) {
+    constexpr int i = I;
+    f[i] = R(weight(i));
+  });
 }
 
 }  // namespace lbm
