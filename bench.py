@@ -90,6 +90,33 @@ def time_forward(lat, f0, steps, warmup):
     return loop_ms, k_ms, wall
 
 
+def time_forward_adjoint(storage, steps, device, checkpoint="all"):
+    """Wall time of forward + hand-written adjoint of dCd/dr (mean Cd over the
+    last 30% of `steps`) for the bench sphere; returns (seconds, dloss/dr)."""
+    import lbm_jax_autodiff_amd as lbm
+    from lbm_jax_autodiff_amd import adjoint as A
+    u_in = 0.05
+    D = 2.0 * 0.5 * NX / 8.0
+    tau = 3.0 * (u_in * D / 100.0) + 0.5
+    dt = torch.float32
+    r = torch.tensor(0.5, dtype=torch.float32, device=device, requires_grad=True)
+    _, solid, q = lbm.smooth_sphere_geometry(NX, NY, NZ, (0.0, 0.0, 0.0), r, 20.0, device=device)
+    f0 = lbm.equilibrium(torch.ones((NX, NY, NZ), dtype=dt, device=device),
+                         torch.zeros((3, NX, NY, NZ), dtype=dt, device=device))
+    area = torch.clamp(lbm.projected_area(solid).to(torch.float64), min=1.0)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    _, _, F = A.rollout(f0, steps, tau=tau, q=q, solid=solid, inlet_velocity=(u_in, 0.0, 0.0), collision="mrt",
+                        guard="pre", guard_velocity=(u_in, 0.0, 0.0), storage=storage, offset="inlet",
+                        checkpoint=checkpoint)
+    loss = A.drag_series(F, u_in, area)[-max(1, int(0.3 * steps)):].mean()
+    (g,) = torch.autograd.grad(loss, r)
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) * 1e-3, float(g), float(loss)
+
+
 def cpu_baseline(sample_steps=2):
     """Oracle (torch CPU float32, the reference's vectorised op structure) on
     the same workload, timed on the host cores over a few steps."""
@@ -109,17 +136,28 @@ def cpu_baseline(sample_steps=2):
     feq_ref = O.equilibrium(rho, uref)
     vel = torch.tensor([u_in, 0.0, 0.0], dtype=torch.float32)
     kw = dict(tau=tau, velocity=vel, q=q, solid=solid, collision="mrt", guard="pre", f_eq_ref=feq_ref)
-    with torch.no_grad():
-        f, _ = O.hot_step(f, **kw)  # warm-up
-        t0 = time.perf_counter()
-        for _ in range(sample_steps):
-            f, fp = O.hot_step(f, **kw)
-            O.momentum_exchange(fp, f, q)
-        dt = time.perf_counter() - t0
+
+    def run(n):
+        nonlocal f
+        with torch.no_grad():
+            t0 = time.perf_counter()
+            for _ in range(n):
+                f, fp = O.hot_step(f, **kw)
+                O.momentum_exchange(fp, f, q)
+            return time.perf_counter() - t0
+
+    run(1)  # warm-up
+    dt = run(sample_steps)
     mlups = NX * NY * NZ * sample_steps / dt / 1e6
+    torch.set_num_threads(1)
+    n1 = max(1, sample_steps // 8)
+    dt1 = run(n1)
+    torch.set_num_threads(cores)
     return {"value": round(mlups, 3), "unit": "MLUPS", "cores": cores, "kind": "port",
+            "one_thread_mlups": round(NX * NY * NZ * n1 / dt1 / 1e6, 3),
             "sample": f"{sample_steps} steps of the 256x128x128 MRT+Bouzidi sphere step incl. momentum exchange "
-                      f"(torch CPU float32, {cores} threads) after 1 warm-up step; {dt:.2f} s"}
+                      f"(oracle, torch CPU float32, {cores} threads) after 1 warm-up step: {dt:.2f} s; "
+                      f"{n1} step(s) on 1 thread: {dt1:.2f} s"}
 
 
 def main():
@@ -128,8 +166,11 @@ def main():
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=40)
     ap.add_argument("--skip-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=16)
+    ap.add_argument("--adj-steps", type=int, default=200)
     ap.add_argument("--no-extras", action="store_true", help="forward float32 only")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 1 GiB device copy (known bytes) for hardware-counter scaling")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build targets one GPU; multi-GPU runs are not part of it "
@@ -143,6 +184,14 @@ def main():
     L.load()
 
     cells = NX * NY * NZ
+    if args.copy_calibration:
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)  # reads 1 GiB, writes 1 GiB
+        torch.cuda.synchronize()
+        del src, dst
+        log("copy calibration: 3 x 1 GiB device copies")
     log(f"workload 256x128x128 MRT+Bouzidi sphere, f32 forward, warmup={args.warmup} steps={args.steps}")
     lat, solid, q, tau = build_lattice("f32", device=dev)
     f0 = initial_state(lat, dev)
@@ -154,13 +203,29 @@ def main():
 
     extras = {}
     if not args.no_extras:
+        del lat
         for st in ("f16", "bf16"):
             lat_h, _, _, _ = build_lattice(st, device=dev)
             lm, km, _ = time_forward(lat_h, f0, max(args.steps // 2, 10), args.warmup // 2)
             extras[f"fwd_{st}_mlups"] = round(cells / (lm * 1e-3) / 1e6, 1)
             extras[f"fwd_{st}_hbm_frac"] = round(76.0 * cells / (km * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+            extras[f"fwd_{st}_kernel_ms"] = round(km, 5)
             log(f"{st} forward: {lm:.4f} ms/step, {extras[f'fwd_{st}_mlups']} MLUPS")
             del lat_h
+        # forward + hand-written adjoint, store-all, dCd/dr of the time-averaged Cd
+        adj_steps = args.adj_steps
+        for st, bpl in (("f32", 380.0), ("f16", 266.0)):
+            time_forward_adjoint(st, 8, dev)  # warm-up (plans, allocator)
+            torch.cuda.empty_cache()
+            sec, g, loss = time_forward_adjoint(st, adj_steps, dev)
+            ml = cells * adj_steps / sec / 1e6
+            extras[f"fwd_adj_{st}_mlups"] = round(ml, 1)
+            extras[f"fwd_adj_{st}_hbm_frac"] = round(bpl * ml * 1e6 / 1e9 / HBM_PEAK_GBS, 4)
+            extras[f"fwd_adj_{st}_dcd_dr"] = g
+            log(f"{st} fwd+adj ({adj_steps} steps, store-all): {sec:.3f} s, {ml:.1f} MLUPS, dCd/dr={g:.6g}")
+            torch.cuda.empty_cache()
+        extras["fwd_adj_steps"] = adj_steps
+        extras["fwd_adj_bytes_per_cell_update"] = {"f32": 380.0, "f16": 266.0}
 
     out = {
         "metric": "fwd & fwd+adjoint MLUPS, % HBM roofline; dCd/dr time at <=1e-3 rel err",

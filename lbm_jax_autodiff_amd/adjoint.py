@@ -127,9 +127,16 @@ class _Checkpoints:
     def stride(self):
         return self.levels[0]
 
+    @property
+    def store_all(self):
+        return self.levels == [1]
+
     def record(self, t, buf):
-        """Called by the forward for every t; keeps what the top level needs."""
-        if t % self.stride == 0:
+        """Called by the forward for every t; keeps what the top level needs.
+        In store-all mode the forward hands over buffers it will not reuse."""
+        if self.store_all:
+            self.top[t] = buf
+        elif t % self.stride == 0:
             self.top[t] = buf.clone()
 
     def _recompute(self, p_start, t0, t1, keep_every):
@@ -190,6 +197,8 @@ class _RolloutFn(torch.autograd.Function):
         cur, nxt = a, b
         for t in range(n_steps - 1):
             ck.record(t, cur)
+            if ck.store_all:
+                nxt = lat.new_state()  # cur is kept: write P_{t+1} to a fresh buffer
             lat.step(cur, nxt, fpart[t] if lat.slots else None)
             cur, nxt = nxt, cur
         ck.record(n_steps - 1, cur)

@@ -118,7 +118,8 @@ def _kat_cd(r, dev):
     """Cd(r) of tests/test_gradients_f64.py:129-207 on the GPU in FP64."""
     nx, ny, nz = 32, 16, 16
     u_in = 0.05
-    tau = 3.0 * (u_in * 8.0 / 50.0) + 0.5
+    D = 32 * 0.5 / 4.0  # tests/test_gradients_f64.py:137
+    tau = 3.0 * (u_in * D / 50.0) + 0.5
     por, solid, q = lbm.smooth_sphere_geometry(nx, ny, nz, (0.0, 0.0, 0.0), r, 20.0, coord_dtype=torch.float64,
                                                out_dtype=torch.float64, disc_floor=1e-20, denom_eps=1e-30,
                                                device=dev)
