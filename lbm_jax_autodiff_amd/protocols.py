@@ -146,9 +146,11 @@ def sphere_cd(nx, ny, nz, n_steps, center_world, radius=0.5, *, init_u=0.0, stor
 def sphere_loss_and_grad(nx, ny, nz, n_steps, center_world, radius=0.5, *, avg_last=600, q_mode="bouzidi",
                          porosity_mode="hard", sharpness=20.0, collision="mrt", re=100.0, u_in=0.05,
                          storage="f32", checkpoint="all", smagorinsky_cs=None, guard="none", init_u=0.0,
-                         offset="inlet", geometry_dtype=None):
+                         offset="inlet", geometry_dtype=None, r_ref=0.5):
     """loss = mean Cd over the last `avg_last` steps and dloss/dr (world units)
-    through q(r) (Bouzidi) and porosity(r) (soft), examples/11:56-167."""
+    through q(r) (Bouzidi) and porosity(r) (soft), examples/11:56-167.  The
+    viscosity comes from the nominal diameter 2 r_ref (examples/11:228-230),
+    so tau does not move with r."""
     dev = _dev()
     dt = _dtype(storage)
     gdt = geometry_dtype or (torch.float64 if storage == "f64" else torch.float32)
@@ -158,7 +160,7 @@ def sphere_loss_and_grad(nx, ny, nz, n_steps, center_world, radius=0.5, *, avg_l
     if q_mode == "halfway":
         q = compute_q_from_solid(solid, nx, ny, nz, 0.5, dtype=dt)
     porosity = por_soft if porosity_mode == "soft" else solid.to(dt) + 0.0 * por_soft
-    D = 2.0 * float(radius) * nx / 8.0
+    D = 2.0 * float(r_ref) * nx / 8.0
     tau = tau_for(u_in, D, re)
     u = torch.zeros((3, nx, ny, nz), dtype=dt, device=dev)
     u[0] = init_u
@@ -178,7 +180,7 @@ def sphere_loss_and_grad(nx, ny, nz, n_steps, center_world, radius=0.5, *, avg_l
     wall = time.perf_counter() - t0
     g = 0.0 if g is None else float(g)
     return {"grid": [nx, ny, nz], "steps": n_steps, "storage": storage, "q_mode": q_mode,
-            "porosity_mode": porosity_mode, "tau": tau, "radius_world": float(radius), "loss": float(loss),
+            "porosity_mode": porosity_mode, "tau": tau, "radius_world": float(radius), "loss": float(loss.detach()),
             "dloss_dr_world": g, "dloss_dr_lattice": g / (nx / 8.0), "checkpoint": str(checkpoint),
             "fwd_s": t_fwd, "fwd_adj_s": wall, "fwd_adj_mlups": nx * ny * nz * n_steps / wall / 1e6}
 
@@ -207,7 +209,8 @@ def _emit(d):
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("config", choices=["c1", "c2", "c3", "c3fd", "c3adam", "c4", "c5"])
+    ap.add_argument("config", choices=["c1", "c2", "c3", "c3fd", "c3fdh", "c3adam", "c4", "c5", "ex10"])
+    ap.add_argument("--grid", default="64")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--storage", default="f32")
     ap.add_argument("--checkpoint", default=None)
@@ -238,6 +241,29 @@ def main(argv=None):
             hw = sphere_loss_and_grad(nx, ny, nz, steps, c, 0.5, storage=args.storage, q_mode="halfway")
             hw["config"] = "c3-halfway-hard"
             _emit(hw)
+    elif args.config == "c3fdh":
+        # central differences of the FP64 kernels at shrinking h
+        nx, ny, nz = 96, 48, 48
+        c = world_center(nx, ny, nz, (24, 24, 24))
+        steps = args.steps or 2000
+        ad = sphere_loss_and_grad(nx, ny, nz, steps, c, 0.5, storage="f64")
+        out = {"config": "c3-fd-sweep", "adjoint_f64": ad["dloss_dr_world"], "fd": {}}
+        for h in (1e-3, 3e-4, 1e-4, 3e-5, 1e-5, 1e-6):
+            lp = sphere_loss_and_grad(nx, ny, nz, steps, c, 0.5 + h, storage="f64")["loss"]
+            lm = sphere_loss_and_grad(nx, ny, nz, steps, c, 0.5 - h, storage="f64")["loss"]
+            out["fd"][str(h)] = (lp - lm) / (2 * h)
+        _emit(out)
+    elif args.config == "ex10":
+        # examples/10_cd_convergence.py:46-121: centre world 0, r = 0.5, init u = U,
+        # pre-collision guard to eq(1, U), Cd snapshot every 500 steps
+        sizes = {"64": (64, 32, 32, 8000), "128": (128, 64, 64, 16000), "256": (256, 128, 128, 28000),
+                 "384": (384, 192, 192, 40000)}
+        nx, ny, nz, steps = sizes[args.grid]
+        res, cd = sphere_cd(nx, ny, nz, args.steps or steps, (0.0, 0.0, 0.0), 0.5, init_u=0.05,
+                            storage=args.storage, guard="pre")
+        res["config"] = "ex10-" + args.grid
+        res["cd_snapshots"] = [round(float(v), 6) for v in cd[499::500].cpu()]
+        _emit(res)
     elif args.config == "c3adam":
         nx, ny, nz = 96, 48, 48
         c = world_center(nx, ny, nz, (24, 24, 24))

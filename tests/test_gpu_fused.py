@@ -92,7 +92,9 @@ def test_guard_resets_only_bad_cells(dev):
     bad = [(3, 2, 1), (7, 4, 3), (10, 8, 5)]
     f0[:, 3, 2, 1] = float("nan")
     f0[:, 7, 4, 3] *= 3.0   # rho ~ 3 > 2
-    f0[1, 10, 8, 5] += 0.5  # |u| > 0.4
+    fast = O.equilibrium(torch.ones((1,), dtype=torch.float64),
+                         torch.tensor([[0.5], [0.0], [0.0]], dtype=torch.float64))
+    f0[:, 10, 8, 5] = fast[:, 0]  # rho = 1, |u| = 0.5 > 0.4
     vel = torch.tensor([0.05, 0.0, 0.0], dtype=torch.float64)
     s_ref, p_ref, _ = _oracle(f0, 1, tau=torch.tensor(0.6, dtype=torch.float64), velocity=vel,
                               q=-torch.ones((19, NX, NY, NZ), dtype=torch.float64), collision="bgk", guard="pre",
