@@ -145,10 +145,11 @@ def sphere_cd(nx, ny, nz, n_steps, center_world, radius=0.5, *, init_u=0.0, stor
 
 def sphere_loss_and_grad(nx, ny, nz, n_steps, center_world, radius=0.5, *, avg_last=600, q_mode="bouzidi",
                          porosity_mode="hard", sharpness=20.0, collision="mrt", re=100.0, u_in=0.05,
-                         storage="f32", checkpoint="all", smagorinsky_cs=None, guard="none", init_u=0.0,
+                         storage="f32", checkpoint="all", smagorinsky_cs=None, guard="none", init_u=0.05,
                          offset="inlet", geometry_dtype=None, r_ref=0.5):
     """loss = mean Cd over the last `avg_last` steps and dloss/dr (world units)
     through q(r) (Bouzidi) and porosity(r) (soft), examples/11:56-167.  The
+    initial state is f_eq(1, (init_u, 0, 0)) (examples/11:147-149 use u_inlet).  The
     viscosity comes from the nominal diameter 2 r_ref (examples/11:228-230),
     so tau does not move with r."""
     dev = _dev()
@@ -277,7 +278,10 @@ def main(argv=None):
     elif args.config == "c4":
         nx, ny, nz = 256, 128, 128
         steps = args.steps or 28000
-        ck = int(args.checkpoint) if args.checkpoint else 100
+        ck = 100
+        if args.checkpoint:
+            parts = [int(v) for v in args.checkpoint.split(",")]
+            ck = parts[0] if len(parts) == 1 else tuple(parts)
         res = sphere_loss_and_grad(nx, ny, nz, steps, (0.0, 0.0, 0.0), 0.5, storage=args.storage,
                                    checkpoint=ck, avg_last=int(0.3 * steps), guard="pre")
         res["config"] = "c4"
