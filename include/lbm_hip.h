@@ -225,6 +225,8 @@ typedef struct lbm_step_desc {
   const void* porosity;     /* N, float (double for LBM_F64): soft bounce */
   const int32_t* link_cell; /* n_links */
   const double* link_q;     /* 18 x n_links */
+  int32_t cells_per_thread; /* dense sweep: 1, 2 or 4 x-consecutive cells per
+                               thread (must divide nx); 0 = widest that does */
 } lbm_step_desc;
 
 /* Number of float64 force partials (3 each) one lbm_step launch writes. */

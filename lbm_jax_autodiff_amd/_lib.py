@@ -41,6 +41,7 @@ class StepDesc(ctypes.Structure):
         ("u_ref", ctypes.c_double * 3), ("offset", ctypes.c_double * 19),
         ("flags", ctypes.c_void_p), ("porosity", ctypes.c_void_p),
         ("link_cell", ctypes.c_void_p), ("link_q", ctypes.c_void_p),
+        ("cells_per_thread", ctypes.c_int32),
     ]
 
 

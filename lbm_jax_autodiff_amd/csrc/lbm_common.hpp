@@ -109,7 +109,28 @@ __host__ __device__ constexpr bool mrt_is_conserved(int k) {
 // Storage formats.  Compute is float for the float32 / half / bfloat16
 // formats and double for the float64 format.  Half formats hold the deviation
 // f_i - off_i from a per-direction offset (the weights w_i by default).
+//   ld / ld_c / st : one element (st is a non-temporal store)
+//   ldv<V> / stv<V>: V consecutive elements at an index that is a multiple of V
 // ---------------------------------------------------------------------------
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf16_bits_to_float(unsigned short raw) {
+  return __uint_as_float(static_cast<unsigned>(raw) << 16);
+}
+__device__ __forceinline__ unsigned short float_to_bf16_bits(float d) {
+  // round to nearest even; NaN stays NaN
+  unsigned u = __float_as_uint(d);
+  if ((u & 0x7f800000u) == 0x7f800000u) return static_cast<unsigned short>((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return static_cast<unsigned short>(u >> 16);
+}
+__device__ __forceinline__ float f16_bits_to_float(unsigned short raw) { return __half2float(__ushort_as_half(raw)); }
+__device__ __forceinline__ unsigned short float_to_f16_bits(float d) { return __half_as_ushort(__float2half_rn(d)); }
+
 struct StoreF32 {
   using T = float;
   using R = float;
@@ -117,6 +138,30 @@ struct StoreF32 {
   __device__ static R ld(const T* p, size_t k, R) { return __builtin_nontemporal_load(p + k); }
   __device__ static R ld_c(const T* p, size_t k, R) { return p[k]; }
   __device__ static void st(T* p, size_t k, R v, R) { __builtin_nontemporal_store(v, p + k); }
+  template <int V>
+  __device__ static void ldv(const T* p, size_t k, R, R* o) {
+    if constexpr (V == 4) {
+      const f32x4 v = *reinterpret_cast<const f32x4*>(p + k);
+      o[0] = v[0]; o[1] = v[1]; o[2] = v[2]; o[3] = v[3];
+    } else if constexpr (V == 2) {
+      const f32x2 v = *reinterpret_cast<const f32x2*>(p + k);
+      o[0] = v[0]; o[1] = v[1];
+    } else {
+      o[0] = p[k];
+    }
+  }
+  template <int V>
+  __device__ static void stv(T* p, size_t k, const R* o, R) {
+    if constexpr (V == 4) {
+      const f32x4 v = {o[0], o[1], o[2], o[3]};
+      __builtin_nontemporal_store(v, reinterpret_cast<f32x4*>(p + k));
+    } else if constexpr (V == 2) {
+      const f32x2 v = {o[0], o[1]};
+      __builtin_nontemporal_store(v, reinterpret_cast<f32x2*>(p + k));
+    } else {
+      __builtin_nontemporal_store(o[0], p + k);
+    }
+  }
 };
 struct StoreF64 {
   using T = double;
@@ -125,47 +170,69 @@ struct StoreF64 {
   __device__ static R ld(const T* p, size_t k, R) { return __builtin_nontemporal_load(p + k); }
   __device__ static R ld_c(const T* p, size_t k, R) { return p[k]; }
   __device__ static void st(T* p, size_t k, R v, R) { __builtin_nontemporal_store(v, p + k); }
-};
-struct StoreF16 {
-  using T = __half;
-  using R = float;
-  static constexpr int tag = LBM_F16DEV;
-  __device__ static R ld(const T* p, size_t k, R off) {
-    const unsigned short raw = __builtin_nontemporal_load(reinterpret_cast<const unsigned short*>(p) + k);
-    return __half2float(__ushort_as_half(raw)) + off;
-  }
-  __device__ static R ld_c(const T* p, size_t k, R off) { return __half2float(p[k]) + off; }
-  __device__ static void st(T* p, size_t k, R v, R off) {
-    const unsigned short raw = __half_as_ushort(__float2half_rn(v - off));
-    __builtin_nontemporal_store(raw, reinterpret_cast<unsigned short*>(p) + k);
-  }
-};
-struct StoreBF16 {
-  using T = __hip_bfloat16;
-  using R = float;
-  static constexpr int tag = LBM_BF16DEV;
-  __device__ static R ld(const T* p, size_t k, R off) {
-    const unsigned short raw = __builtin_nontemporal_load(reinterpret_cast<const unsigned short*>(p) + k);
-    return __uint_as_float(static_cast<unsigned>(raw) << 16) + off;
-  }
-  __device__ static R ld_c(const T* p, size_t k, R off) {
-    const unsigned short raw = reinterpret_cast<const unsigned short*>(p)[k];
-    return __uint_as_float(static_cast<unsigned>(raw) << 16) + off;
-  }
-  __device__ static void st(T* p, size_t k, R v, R off) {
-    // round to nearest even on the deviation
-    const float d = v - off;
-    unsigned u = __float_as_uint(d);
-    unsigned short raw;
-    if ((u & 0x7f800000u) == 0x7f800000u) {
-      raw = static_cast<unsigned short>((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
-    } else {
-      u += 0x7fffu + ((u >> 16) & 1u);
-      raw = static_cast<unsigned short>(u >> 16);
+  template <int V>
+  __device__ static void ldv(const T* p, size_t k, R, R* o) {
+#pragma unroll
+    for (int h = 0; h < V; h += 2) {
+      if constexpr (V >= 2) {
+        const f64x2 v = *reinterpret_cast<const f64x2*>(p + k + h);
+        o[h] = v[0];
+        o[h + 1] = v[1];
+      }
     }
-    __builtin_nontemporal_store(raw, reinterpret_cast<unsigned short*>(p) + k);
+    if constexpr (V == 1) o[0] = p[k];
+  }
+  template <int V>
+  __device__ static void stv(T* p, size_t k, const R* o, R) {
+#pragma unroll
+    for (int h = 0; h < V; h += 2) {
+      if constexpr (V >= 2) {
+        const f64x2 v = {o[h], o[h + 1]};
+        __builtin_nontemporal_store(v, reinterpret_cast<f64x2*>(p + k + h));
+      }
+    }
+    if constexpr (V == 1) __builtin_nontemporal_store(o[0], p + k);
   }
 };
+template <bool BF>
+struct StoreHalfDev {
+  using T = typename std::conditional<BF, __hip_bfloat16, __half>::type;
+  using R = float;
+  static constexpr int tag = BF ? LBM_BF16DEV : LBM_F16DEV;
+  __device__ static float dec(unsigned short raw) { return BF ? bf16_bits_to_float(raw) : f16_bits_to_float(raw); }
+  __device__ static unsigned short enc(float d) { return BF ? float_to_bf16_bits(d) : float_to_f16_bits(d); }
+  __device__ static const unsigned short* bits(const T* p) { return reinterpret_cast<const unsigned short*>(p); }
+  __device__ static unsigned short* bits(T* p) { return reinterpret_cast<unsigned short*>(p); }
+  __device__ static R ld(const T* p, size_t k, R off) { return dec(__builtin_nontemporal_load(bits(p) + k)) + off; }
+  __device__ static R ld_c(const T* p, size_t k, R off) { return dec(bits(p)[k]) + off; }
+  __device__ static void st(T* p, size_t k, R v, R off) { __builtin_nontemporal_store(enc(v - off), bits(p) + k); }
+  template <int V>
+  __device__ static void ldv(const T* p, size_t k, R off, R* o) {
+    if constexpr (V == 4) {
+      const u16x4 v = *reinterpret_cast<const u16x4*>(bits(p) + k);
+      o[0] = dec(v[0]) + off; o[1] = dec(v[1]) + off; o[2] = dec(v[2]) + off; o[3] = dec(v[3]) + off;
+    } else if constexpr (V == 2) {
+      const u16x2 v = *reinterpret_cast<const u16x2*>(bits(p) + k);
+      o[0] = dec(v[0]) + off; o[1] = dec(v[1]) + off;
+    } else {
+      o[0] = ld_c(p, k, off);
+    }
+  }
+  template <int V>
+  __device__ static void stv(T* p, size_t k, const R* o, R off) {
+    if constexpr (V == 4) {
+      const u16x4 v = {enc(o[0] - off), enc(o[1] - off), enc(o[2] - off), enc(o[3] - off)};
+      __builtin_nontemporal_store(v, reinterpret_cast<u16x4*>(bits(p) + k));
+    } else if constexpr (V == 2) {
+      const u16x2 v = {enc(o[0] - off), enc(o[1] - off)};
+      __builtin_nontemporal_store(v, reinterpret_cast<u16x2*>(bits(p) + k));
+    } else {
+      st(p, k, o[0], off);
+    }
+  }
+};
+using StoreF16 = StoreHalfDev<false>;
+using StoreBF16 = StoreHalfDev<true>;
 
 // Fast unsigned division by a run-time constant (round-up multiplier method
 // of Granlund & Montgomery, 1994): q = (mulhi(n, m) + n) >> s with

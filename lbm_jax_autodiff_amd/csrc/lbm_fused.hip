@@ -140,7 +140,117 @@ __device__ __forceinline__ void store_cell(const StepArgs<typename S::R>& a, typ
   });
 }
 
-template <class S, int COLL, int MODE>
+// Dense sweep with V x-consecutive cells per thread (nx % V == 0, so the V
+// cells share one row).  Every population is read with one aligned V-wide
+// load from the source row; the one-cell x shift of the e_x = +-1 pulls is
+// taken from the neighbouring lane's block (lane shuffle), with one
+// predicated load at wave edges and the cell's own value at the x faces.
+template <class S, int COLL, int MODE, int V>
+__device__ __forceinline__ void dense_vec(const StepArgs<typename S::R>& a, const typename S::T* __restrict__ src,
+                                          typename S::T* __restrict__ dst, uint32_t t) {
+  using R = typename S::R;
+  const size_t N = a.N;
+  const uint32_t c0 = t * V;
+  if (c0 >= a.N) return;
+  int x0, y, z;
+  cell_coords(a, c0, x0, y, z);
+  const int lane = threadIdx.x & 63;
+  const bool row_first = (x0 == 0), row_last = (x0 + V == a.nx);
+  R s[V][Q];
+  if constexpr (MODE == LBM_MODE_COLLIDE) {
+    sfor<Q>([&](auto I) {
+      constexpr int i = I;
+      R blk[V];
+      S::template ldv<V>(src, i * N + c0, a.off[i], blk);
+#pragma unroll
+      for (int j = 0; j < V; ++j) s[j][i] = blk[j];
+    });
+  } else {
+    const Rows rows = make_rows(a, y, z);
+    sfor<Q>([&](auto I) {
+      constexpr int i = I;
+      constexpr int ex = EX[i], ey = EY[i], ez = EZ[i];
+      // aligned block of the source row (x not shifted), or the own row when
+      // a clamped y/z face keeps the cell's own values
+      uint32_t base = c0;
+      bool row_ok = true;
+      if constexpr (ey == 1) { row_ok = row_ok && rows.ym_ok; base += rows.ym; }
+      else if constexpr (ey == -1) { row_ok = row_ok && rows.yp_ok; base += rows.yp; }
+      if constexpr (ez == 1) { row_ok = row_ok && rows.zm_ok; base += rows.zm; }
+      else if constexpr (ez == -1) { row_ok = row_ok && rows.zp_ok; base += rows.zp; }
+      if (!row_ok) base = c0;
+      R blk[V];
+      S::template ldv<V>(src, i * N + base, a.off[i], blk);
+      if constexpr (ex == 0) {
+#pragma unroll
+        for (int j = 0; j < V; ++j) s[j][i] = blk[j];
+      } else if constexpr (ex == 1) {
+        // cell x0 + j pulls x0 + j - 1
+        R prev = __shfl_up(blk[V - 1], 1, 64);
+        if (row_ok && !row_first && lane == 0) prev = S::ld_c(src, i * N + base - 1, a.off[i]);
+        if (row_first && row_ok) prev = S::ld_c(src, i * N + c0, a.off[i]);  // x = 0 keeps its own value
+        s[0][i] = row_ok ? prev : blk[0];
+#pragma unroll
+        for (int j = 1; j < V; ++j) s[j][i] = row_ok ? blk[j - 1] : blk[j];
+      } else {
+        // cell x0 + j pulls x0 + j + 1
+        R next = __shfl_down(blk[0], 1, 64);
+        if (row_ok && !row_last && lane == 63) next = S::ld_c(src, i * N + base + V, a.off[i]);
+        if (row_last && row_ok) next = S::ld_c(src, i * N + c0 + V - 1, a.off[i]);  // x = nx-1 keeps its own
+#pragma unroll
+        for (int j = 0; j < V - 1; ++j) s[j][i] = row_ok ? blk[j + 1] : blk[j];
+        s[V - 1][i] = row_ok ? next : blk[V - 1];
+      }
+    });
+  }
+  // per-cell flags (bit0 solid, bit1 link cell owned by a link workgroup)
+  uint32_t fl = 0;
+  if (a.flags) {
+    if constexpr (V == 4) fl = *reinterpret_cast<const uint32_t*>(a.flags + c0);
+    else if constexpr (V == 2) fl = *reinterpret_cast<const uint16_t*>(a.flags + c0);
+    else fl = a.flags[c0];
+  }
+  R phi[V];
+  if (a.bounce == LBM_BOUNCE_SOFT) {
+#pragma unroll
+    for (int j = 0; j < V; ++j) phi[j] = a.phi[c0 + j];
+  }
+#pragma unroll
+  for (int j = 0; j < V; ++j) {
+    const uint32_t fj = (fl >> (8 * j)) & 0xffu;
+    if constexpr (MODE != LBM_MODE_COLLIDE) boundaries(a, s[j], x0 + j);
+    if constexpr (MODE != LBM_MODE_STREAM) {
+      if (a.guard == LBM_GUARD_PRE && guard_bad(s[j])) set_feq_unit(s[j], a.uref[0], a.uref[1], a.uref[2]);
+      R tau = a.tau;
+      if (a.smag) tau = smag_tau(s[j], a.tau, a.cs);
+      const R omega = R(1) / tau;
+      if constexpr (COLL == LBM_COLL_MRT) collide_mrt(s[j], omega);
+      else collide_bgk(s[j], omega);
+      if (a.bounce == LBM_BOUNCE_HARD) {
+        if (fj & 1u) reverse_all(s[j]);
+      } else if (a.bounce == LBM_BOUNCE_SOFT) {
+        blend_opposite(s[j], phi[j]);
+      }
+    }
+  }
+  // link cells are written by the link workgroups (dense MODE_COLLIDE writes all)
+  const bool has_link = (MODE != LBM_MODE_COLLIDE) && (fl & 0x02020202u);
+  sfor<Q>([&](auto I) {
+    constexpr int i = I;
+    R o[V];
+#pragma unroll
+    for (int j = 0; j < V; ++j) o[j] = s[j][i];
+    if (!has_link) {
+      S::template stv<V>(dst, i * N + c0, o, a.off[i]);
+    } else {
+#pragma unroll
+      for (int j = 0; j < V; ++j)
+        if (!((fl >> (8 * j)) & 2u)) S::st(dst, i * N + c0 + j, o[j], a.off[i]);
+    }
+  });
+}
+
+template <class S, int COLL, int MODE, int V>
 __global__ __launch_bounds__(kStepBlock) void k_step(StepArgs<typename S::R> a,
                                                      const typename S::T* __restrict__ src,
                                                      typename S::T* __restrict__ dst,
@@ -224,6 +334,10 @@ __global__ __launch_bounds__(kStepBlock) void k_step(StepArgs<typename S::R> a,
   // ---------------- dense sweep ----------------
   const uint32_t first = (MODE == LBM_MODE_COLLIDE) ? 0u : uint32_t(a.nlb);
   const uint32_t tile = xcd_swizzle(bid - first, uint32_t(a.ndb));
+  if constexpr (V > 1) {
+    dense_vec<S, COLL, MODE, V>(a, src, dst, tile * kStepBlock + threadIdx.x);
+    return;
+  }
   const uint32_t c = tile * kStepBlock + threadIdx.x;
   if (c >= a.N) return;
   R s[Q];
@@ -314,6 +428,16 @@ int make_args(const lbm_step_desc* d, int mode, StepArgs<R>& a) {
   return LBM_OK;
 }
 
+// Cells per thread of the dense sweep: the requested value if it divides nx,
+// else the widest of 4, 2, 1 that does.
+inline int choose_vec(const lbm_step_desc* d) {
+  const int want = d->cells_per_thread;
+  if (want == 1 || want == 2 || want == 4) return (d->nx % want == 0) ? want : 1;
+  if (d->nx % 4 == 0) return 4;
+  if (d->nx % 2 == 0) return 2;
+  return 1;
+}
+
 template <class S>
 int launch_step(const lbm_step_desc* d, int mode, const void* src, void* dst, double* fpart, hipStream_t s) {
   using R = typename S::R;
@@ -321,22 +445,32 @@ int launch_step(const lbm_step_desc* d, int mode, const void* src, void* dst, do
   const int rc = make_args(d, mode, a);
   if (rc) return rc;
   if (!src || !dst || src == dst) return LBM_E_ARG;
+  const int V = choose_vec(d);
+  const long long cells_per_block = 1LL * kStepBlock * V;
+  a.ndb = int((1LL * a.N + cells_per_block - 1) / cells_per_block);
   const unsigned grid = unsigned(a.nlb + a.ndb);
   const auto* in = static_cast<const typename S::T*>(src);
   auto* out = static_cast<typename S::T*>(dst);
-#define LBM_STEP_LAUNCH(COLL, MODE) \
-  hipLaunchKernelGGL((k_step<S, COLL, MODE>), grid, kStepBlock, 0, s, a, in, out, fpart)
+#define LBM_STEP_LAUNCH(COLL, MODE, VV) \
+  hipLaunchKernelGGL((k_step<S, COLL, MODE, VV>), grid, kStepBlock, 0, s, a, in, out, fpart)
+#define LBM_STEP_V(COLL, MODE)                                      \
+  do {                                                              \
+    if (V == 4) LBM_STEP_LAUNCH(COLL, MODE, 4);                     \
+    else if (V == 2) LBM_STEP_LAUNCH(COLL, MODE, 2);                \
+    else LBM_STEP_LAUNCH(COLL, MODE, 1);                            \
+  } while (0)
   if (a.coll == LBM_COLL_MRT) {
-    if (mode == LBM_MODE_FULL) LBM_STEP_LAUNCH(LBM_COLL_MRT, LBM_MODE_FULL);
-    else if (mode == LBM_MODE_COLLIDE) LBM_STEP_LAUNCH(LBM_COLL_MRT, LBM_MODE_COLLIDE);
-    else if (mode == LBM_MODE_STREAM) LBM_STEP_LAUNCH(LBM_COLL_MRT, LBM_MODE_STREAM);
+    if (mode == LBM_MODE_FULL) LBM_STEP_V(LBM_COLL_MRT, LBM_MODE_FULL);
+    else if (mode == LBM_MODE_COLLIDE) LBM_STEP_V(LBM_COLL_MRT, LBM_MODE_COLLIDE);
+    else if (mode == LBM_MODE_STREAM) LBM_STEP_V(LBM_COLL_MRT, LBM_MODE_STREAM);
     else return LBM_E_ARG;
   } else {
-    if (mode == LBM_MODE_FULL) LBM_STEP_LAUNCH(LBM_COLL_BGK, LBM_MODE_FULL);
-    else if (mode == LBM_MODE_COLLIDE) LBM_STEP_LAUNCH(LBM_COLL_BGK, LBM_MODE_COLLIDE);
-    else if (mode == LBM_MODE_STREAM) LBM_STEP_LAUNCH(LBM_COLL_BGK, LBM_MODE_STREAM);
+    if (mode == LBM_MODE_FULL) LBM_STEP_V(LBM_COLL_BGK, LBM_MODE_FULL);
+    else if (mode == LBM_MODE_COLLIDE) LBM_STEP_V(LBM_COLL_BGK, LBM_MODE_COLLIDE);
+    else if (mode == LBM_MODE_STREAM) LBM_STEP_V(LBM_COLL_BGK, LBM_MODE_STREAM);
     else return LBM_E_ARG;
   }
+#undef LBM_STEP_V
 #undef LBM_STEP_LAUNCH
   return hipGetLastError() == hipSuccess ? LBM_OK : LBM_E_LAUNCH;
 }

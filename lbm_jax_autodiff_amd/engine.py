@@ -100,12 +100,15 @@ class Lattice:
     (examples/10:71-75, reset to eq(1, guard_velocity)).
     ``storage`` f32 / f64 / f16 / bf16 (half formats store f_i - offset_i
     with float32 arithmetic); ``offset`` "weights" (w_i) or "inlet"
-    (f_eq(1, inlet_velocity)) or 19 numbers.
+    (f_eq(1, inlet_velocity)) or 19 numbers.  ``cells_per_thread`` (1, 2
+    or 4; 0 = widest dividing nx) sets how many x-consecutive cells one thread
+    of the dense sweep updates with vector loads.
     """
 
     def __init__(self, nx, ny, nz, *, tau, inlet_velocity=(0.05, 0.0, 0.0), rho_out=1.0, collision="bgk",
                  smagorinsky_cs=None, solid=None, porosity=None, q=None, guard="none", guard_velocity=None,
-                 periodic_yz=True, storage="f32", offset="weights", device=None, dqdr=None):
+                 periodic_yz=True, storage="f32", offset="weights", device=None, dqdr=None,
+                 cells_per_thread=0):
         if storage not in _STORAGE:
             raise ValueError(f"storage must be one of {sorted(_STORAGE)}")
         if guard not in _GUARD:
@@ -176,6 +179,7 @@ class Lattice:
         d.porosity = L.ptr(self.porosity_storage)
         d.link_cell = L.ptr(self.links.cell)
         d.link_q = L.ptr(self.links.q)
+        d.cells_per_thread = int(cells_per_thread)
         self.desc = d
         self.slots = int(L.load().lbm_step_force_slots(ctypes.byref(d)))
 

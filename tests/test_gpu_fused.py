@@ -12,7 +12,9 @@ from _helpers import near_eq_state, random_q, cpu, maxabs
 
 pytestmark = pytest.mark.gpu
 
-NX, NY, NZ = 14, 9, 6
+# nx = 20: at 4 (2) cells per thread a row is 5 (10) lanes, so rows straddle
+# wavefront boundaries and both the lane-shuffle and the wave-edge loads run
+NX, NY, NZ = 20, 9, 6
 
 
 def _oracle(f0, steps, **kw):
@@ -21,8 +23,8 @@ def _oracle(f0, steps, **kw):
 
 
 def _run(dev, f0, steps, storage, collision, bounce, guard, smag, periodic, q, solid, phi, tau=0.6,
-         vel=(0.05, 0.0, 0.0), uref=None):
-    lat = Lattice(NX, NY, NZ, tau=tau, inlet_velocity=vel, collision=collision,
+         vel=(0.05, 0.0, 0.0), uref=None, vec=0):
+    lat = Lattice(NX, NY, NZ, tau=tau, inlet_velocity=vel, collision=collision, cells_per_thread=vec,
                   smagorinsky_cs=0.1 if smag else None,
                   solid=solid.to(dev) if (bounce == "hard") else None,
                   porosity=phi.to(dev) if bounce == "soft" else None,
@@ -33,11 +35,11 @@ def _run(dev, f0, steps, storage, collision, bounce, guard, smag, periodic, q, s
 
 
 CASES = list(itertools.product(["bgk", "mrt"], ["none", "hard", "soft"], ["none", "post", "pre"], [False, True],
-                               [True, False], [0.1, 0.0]))
+                               [True, False], [0.1, 0.0], [1, 2, 4]))
 
 
-@pytest.mark.parametrize("collision,bounce,guard,smag,periodic,p_wall", CASES)
-def test_fused_matches_oracle_f64(dev, collision, bounce, guard, smag, periodic, p_wall):
+@pytest.mark.parametrize("collision,bounce,guard,smag,periodic,p_wall,vec", CASES)
+def test_fused_matches_oracle_f64(dev, collision, bounce, guard, smag, periodic, p_wall, vec):
     steps = 6
     f0 = near_eq_state(NX, NY, NZ, amp=2e-3)
     q = random_q(NX, NY, NZ, seed=4, p_wall=p_wall)
@@ -55,33 +57,36 @@ def test_fused_matches_oracle_f64(dev, collision, bounce, guard, smag, periodic,
         solid=solid if bounce == "hard" else None, porosity=phi if bounce == "soft" else None,
         collision=collision, smagorinsky_cs=0.1 if smag else None, periodic_yz=periodic, guard=guard,
         f_eq_ref=feq_ref)
-    s, p, fh = _run(dev, f0, steps, "f64", collision, bounce, guard, smag, periodic, q, solid, phi, uref=uref)
+    s, p, fh = _run(dev, f0, steps, "f64", collision, bounce, guard, smag, periodic, q, solid, phi, uref=uref,
+                    vec=vec)
     assert maxabs(s, s_ref) < 1e-12
     assert maxabs(p, p_ref) < 1e-12
     assert maxabs(fh, f_ref) < 1e-11
 
 
+@pytest.mark.parametrize("vec", [1, 4])
 @pytest.mark.parametrize("collision", ["bgk", "mrt"])
-def test_fused_f32_close_to_f64_oracle(dev, collision):
+def test_fused_f32_close_to_f64_oracle(dev, collision, vec):
     steps = 20
     f0 = near_eq_state(NX, NY, NZ, amp=2e-3)
     q = random_q(NX, NY, NZ, seed=4, p_wall=0.1)
     vel = torch.tensor([0.05, 0.0, 0.0], dtype=torch.float64)
     s_ref, p_ref, f_ref = _oracle(f0, steps, tau=torch.tensor(0.6, dtype=torch.float64), velocity=vel, q=q,
                                   collision=collision)
-    s, p, fh = _run(dev, f0, steps, "f32", collision, "none", "none", False, True, q, None, None)
+    s, p, fh = _run(dev, f0, steps, "f32", collision, "none", "none", False, True, q, None, None, vec=vec)
     assert maxabs(s, s_ref) < 5e-6
     assert maxabs(p, p_ref) < 5e-6
     assert maxabs(fh, f_ref) < 1e-4
 
 
+@pytest.mark.parametrize("vec", [1, 2, 4])
 @pytest.mark.parametrize("storage,tol", [("f16", 3e-4), ("bf16", 3e-3)])
-def test_half_storage_close_to_f32(dev, storage, tol):
+def test_half_storage_close_to_f32(dev, storage, tol, vec):
     steps = 10
     f0 = near_eq_state(NX, NY, NZ, amp=2e-3)
-    q = random_q(NX, NY, NZ, seed=4, p_wall=0.1)
-    s32, p32, f32 = _run(dev, f0, steps, "f32", "mrt", "none", "none", False, True, q, None, None)
-    sh, ph, fh = _run(dev, f0, steps, storage, "mrt", "none", "none", False, True, q, None, None)
+    q = random_q(NX, NY, NZ, seed=4, p_wall=0.02)
+    s32, p32, f32 = _run(dev, f0, steps, "f32", "mrt", "none", "none", False, True, q, None, None, vec=1)
+    sh, ph, fh = _run(dev, f0, steps, storage, "mrt", "none", "none", False, True, q, None, None, vec=vec)
     assert maxabs(sh, s32) < tol
     assert maxabs(ph, p32) < tol
 
