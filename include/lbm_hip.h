@@ -226,8 +226,21 @@ typedef struct lbm_step_desc {
   const int32_t* link_cell; /* n_links */
   const double* link_q;     /* 18 x n_links */
   int32_t cells_per_thread; /* dense sweep: 1, 2 or 4 x-consecutive cells per
-                               thread (must divide nx); 0 = widest that does */
+                               thread; 2 and 4 apply when they divide nx
+                               and pop_stride, anything else (0 = default)
+                               means 1 */
+  int64_t pop_stride;       /* elements between population i and i+1 of every
+                               state and cotangent buffer the fused entry
+                               points read or write (the storage is
+                               [19][pop_stride], cells x-fastest); 0 = N.
+                               lbm_state_stride gives the recommended value */
 } lbm_step_desc;
+
+/* Recommended pop_stride (>= N) for the grid of d: a padded stride that keeps
+ * the 19 population streams of a cell out of the same HBM channels.  No
+ * counterpart in the reference (its state is a dense jnp array); 0 on bad
+ * dimensions. */
+int64_t lbm_state_stride(const lbm_step_desc* d);
 
 /* Number of float64 force partials (3 each) one lbm_step launch writes. */
 int lbm_step_force_slots(const lbm_step_desc* d);
@@ -261,7 +274,7 @@ typedef struct lbm_adjoint_desc {
   double* phi_acc;         /* N: += dL/dporosity (soft bounce; may be null) */
   double* u_acc;           /* 3 x (ny*nz): += dL/du_in per inlet cell (may be null) */
   int32_t n_targets;       /* size of the scatter plan */
-  const int32_t* tgt_index;/* n_targets: lambda entry i*N + cell */
+  const int32_t* tgt_index;/* n_targets: lambda entry i*pop_stride + cell */
   const int32_t* tgt_start;/* n_targets + 1: CSR offsets into src/coef */
   const int32_t* src;      /* >= 0: mu_link index i*n_links + b; < 0: force term of direction -1-src */
   const double* coef;      /* multiplier of mu_link[src] */
@@ -283,7 +296,7 @@ int lbm_adjoint_scatter(const lbm_step_desc* d, const lbm_adjoint_desc* a, void*
                         const double* gF, lbm_stream_t stream);
 
 /* Convert between storage formats (e.g. float32 <-> half deviations) for a
- * full 19N state. */
+ * full state; src and dst both use d->pop_stride. */
 int lbm_convert_state(const lbm_step_desc* d, int src_dtype, const void* src,
                       int dst_dtype, void* dst, lbm_stream_t stream);
 

@@ -41,7 +41,7 @@ class StepDesc(ctypes.Structure):
         ("u_ref", ctypes.c_double * 3), ("offset", ctypes.c_double * 19),
         ("flags", ctypes.c_void_p), ("porosity", ctypes.c_void_p),
         ("link_cell", ctypes.c_void_p), ("link_q", ctypes.c_void_p),
-        ("cells_per_thread", ctypes.c_int32),
+        ("cells_per_thread", ctypes.c_int32), ("pop_stride", ctypes.c_int64),
     ]
 
 
@@ -79,6 +79,7 @@ _SIGS = {
     "lbm_links_build": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
                                 c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
     "lbm_step_force_slots": (c_int, [ctypes.POINTER(StepDesc)]),
+    "lbm_state_stride": (ctypes.c_int64, [ctypes.POINTER(StepDesc)]),
     "lbm_step": (c_int, [ctypes.POINTER(StepDesc), c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "lbm_force_reduce": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "lbm_convert_state": (c_int, [ctypes.POINTER(StepDesc), c_int, c_void_p, c_int, c_void_p, c_void_p]),

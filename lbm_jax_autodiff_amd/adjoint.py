@@ -57,7 +57,8 @@ def scatter_plan(lat: Lattice):
     n = lat.links.n
     if n == 0:
         return None
-    nx, ny, nz, N = lat.nx, lat.ny, lat.nz, lat.N
+    nx, ny, nz = lat.nx, lat.ny, lat.nz
+    N = getattr(lat, "P", lat.N)  # population stride of the cotangent buffers
     cell = lat.links.cell.detach().to("cpu").numpy().astype(np.int64)
     q = lat.links.q.detach().to("cpu").numpy()  # (18, n)
     x = cell % nx
@@ -137,7 +138,7 @@ class _Checkpoints:
         if self.store_all:
             self.top[t] = buf
         elif t % self.stride == 0:
-            self.top[t] = buf.clone()
+            self.top[t] = self.lat.clone_state(buf)
 
     def _recompute(self, p_start, t0, t1, keep_every):
         """{t: P_t} for t = t0 and every t0 < t < t1 with t % keep_every == 0, from P_t0."""
@@ -157,7 +158,7 @@ class _Checkpoints:
                 dst = sa if cur is not sa else sb
                 lat.step(cur, dst)
                 if (t + 1) % keep_every == 0:
-                    out[t + 1] = dst.clone()
+                    out[t + 1] = lat.clone_state(dst)
                 cur = dst
         return out
 
@@ -206,7 +207,7 @@ class _RolloutFn(torch.autograd.Function):
         lat.stream(cur, s_final, fpart[n_steps - 1] if lat.slots else None)
         forces = lat.reduce_forces(fpart, n_steps) if lat.slots else torch.zeros(
             (n_steps, 3), dtype=torch.float64, device=lat.device)
-        p_last = cur.clone() if lat.storage in ("f16", "bf16") else cur
+        p_last = lat.clone_state(cur) if lat.storage in ("f16", "bf16") else cur
         ctx.lat, ctx.ck, ctx.s0 = lat, ck, s0
         ctx.cfg = cfg
         ctx.has_q = q is not None
@@ -264,10 +265,13 @@ class _RolloutFn(torch.autograd.Function):
         dptr = ctypes.byref(lat.desc)
         aptr = ctypes.byref(ad)
 
-        lam_a = torch.empty((19, nz, ny, nx), dtype=real, device=dev)
-        lam_b = torch.empty_like(lam_a)
-        g_in = F.pop_storage(gS.detach()).to(real) if gS is not None else torch.zeros_like(lam_a)
-        g_in = g_in.contiguous()
+        lam_a = lat.new_state(real)
+        lam_b = lat.new_state(real)
+        g_in = lat.new_state(real)
+        if gS is not None:
+            g_in.copy_(gS.detach().permute(0, 3, 2, 1))
+        else:
+            g_in.zero_()
         it = ck.descending()
         t, p_t = next(it)
         assert t == n_steps - 1
@@ -275,7 +279,7 @@ class _RolloutFn(torch.autograd.Function):
                L.ptr(gF[t]), stream)
         L.call("lbm_adjoint_scatter", dptr, aptr, L.ptr(lam_a), L.ptr(gF[t]), stream)
         if gP is not None:
-            lam_a += F.pop_storage(gP.detach()).to(real)
+            lam_a += gP.detach().permute(0, 3, 2, 1).to(real)
         cur, nxt = lam_a, lam_b
         for t, p_t in it:
             L.call("lbm_step_adjoint", dptr, aptr, L.LBM_MODE_FULL, L.ptr(p_t), L.ptr(cur), L.ptr(nxt),

@@ -1,6 +1,7 @@
 // Hand-written adjoint of the fused D3Q19 step (reverse mode over one step).
 //
-// Given P_t and lambda_{t+1} = dL/dP_{t+1}, one launch recomputes each cell's
+// Given P_t and lambda_{t+1} = dL/dP_{t+1}, one step (a launch for the link
+// cells, then one for the dense grid) recomputes each cell's
 // forward step (pull, Bouzidi, Zou-He, guards, collision, bounce) and pulls
 // the cotangent back to mu = dL/d(pulled values).  Pull streaming is a
 // bijection once the x faces and the y/z axes wrap: entry (y, i) of P_t is
@@ -18,15 +19,17 @@
 namespace lbm {
 
 constexpr int kAdjBlock = 256;
+constexpr int kAdjLinkBlock = 64;
 
 template <typename R>
 struct AdjArgs {
   // forward configuration (same meaning as in the forward kernel)
   int nx, ny, nz;
-  uint32_t N;
+  uint32_t N;  // cells
+  uint32_t P;  // elements between consecutive populations (state and cotangent buffers)
   FastDiv dnx, dny;
   int smag, bounce, guard, n_links, nlb, ndb;
-  R tau, cs;
+  R tau, omega, cs;
   R uin[3];
   R rho_out;
   R uref[3];
@@ -77,13 +80,18 @@ __device__ __forceinline__ bool x_fallback(int x, int nx) {
   else return false;
 }
 
-// Pulled values of cell x (normal pulls and, for link cells, Bouzidi).
+// Pulled values of cell x (normal pulls and, for link cells, Bouzidi).  For
+// link cells qv holds q of the 18 directions (qv[0] unused; loaded by the
+// caller together with the cell index), own receives the cell's
+// P_t and v2[i] the value loaded for direction i (the second neighbour
+// P(x + e_i)_opp where 0 <= q_opp < 1/2); every load is unconditional so a link cell costs three
+// dependent round trips instead of one per direction.
 template <class S, bool LINK>
 __device__ __forceinline__ void pull_cell(const AdjArgs<typename S::R>& a, const typename S::T* __restrict__ P,
                                           uint32_t c, int x, int y, int z, int b, typename S::R* s,
-                                          typename S::R* own) {
+                                          typename S::R* own, const typename S::R* qv, typename S::R* v2) {
   using R = typename S::R;
-  const size_t N = a.N;
+  const size_t N = a.P;
   if constexpr (LINK) {
     sfor<Q>([&](auto I) {
       constexpr int i = I;
@@ -97,25 +105,23 @@ __device__ __forceinline__ void pull_cell(const AdjArgs<typename S::R>& a, const
     constexpr int i = I;
     if constexpr (i != 0) {
       constexpr int io = opp(i);
-      bool link = false;
-      R qo = R(-1);
+      const size_t normal = i * N + (x_fallback<i>(x, a.nx) ? c : owner_target<i>(a, x, y, z));
       if constexpr (LINK) {
-        qo = R(a.link_q[size_t(io - 1) * a.n_links + b]);
-        link = qo >= R(0);
-      }
-      if (link) {
-        if constexpr (LINK) {
-          if (qo >= R(0.5)) {
-            const R inv2q = R(1) / (R(2) * fmin(fmax(qo, R(0.5)), R(1)));
-            s[i] = inv2q * own[io] + (R(1) - inv2q) * own[i];
-          } else {
-            const R twoq = R(2) * fmin(fmax(qo, R(0)), R(0.5));
-            s[i] = twoq * own[io] + (R(1) - twoq) * S::ld_c(P, io * N + next_cell<i>(a, x, y, z), a.off[io]);
-          }
+        const R qo = qv[io];
+        const bool low = (qo >= R(0)) & (qo < R(0.5));
+        const R v = S::ld_c(P, low ? io * N + next_cell<i>(a, x, y, z) : normal, low ? a.off[io] : a.off[i]);
+        if (qo >= R(0.5)) {
+          const R inv2q = R(1) / (R(2) * fmin(qo, R(1)));
+          s[i] = inv2q * own[io] + (R(1) - inv2q) * own[i];
+        } else if (low) {
+          const R twoq = R(2) * qo;
+          s[i] = twoq * own[io] + (R(1) - twoq) * v;
+        } else {
+          s[i] = v;
         }
+        v2[i] = v;
       } else {
-        const uint32_t src = x_fallback<i>(x, a.nx) ? c : owner_target<i>(a, x, y, z);
-        s[i] = S::ld_c(P, i * N + src, a.off[i]);
+        s[i] = S::ld_c(P, normal, a.off[i]);
       }
     }
   });
@@ -129,28 +135,32 @@ __device__ __forceinline__ void apply_zou_he(const AdjArgs<R>& a, R* s, int x) {
 
 // g <- (pre-guard, collide, bounce)^T g at the collision input s (the cell's
 // S_{t+1}); s is replaced by the pre-guard reset state where the guard fires.
-// Accumulates dL/dporosity.  Returns true when the pre guard fired.
+// Accumulates dL/dporosity unless `skip`.  fl = the cell's flags.  Returns
+// true when the pre guard fired.
 template <typename R, int COLL>
-__device__ __forceinline__ bool collide_transpose(const AdjArgs<R>& a, R* s, R* g, uint32_t c) {
-  bool pre_reset = false;
-  if (a.guard == LBM_GUARD_PRE && guard_bad(s)) {
-    pre_reset = true;
+__device__ __forceinline__ bool collide_transpose(const AdjArgs<R>& a, R* s, R* g, uint32_t c, uint32_t fl,
+                                                  bool skip) {
+  Pairs<R> ps = to_pairs(s);
+  Macro<R> m = macro(ps);
+  const bool pre_reset = (a.guard == LBM_GUARD_PRE) && guard_bad(m);
+  if (pre_reset) {
     set_feq_unit(s, a.uref[0], a.uref[1], a.uref[2]);
+    ps = to_pairs(s);
+    m = macro(ps);
   }
-  const R tau_e = a.smag ? smag_tau(s, a.tau, a.cs) : a.tau;
-  const R omega = R(1) / tau_e;
+  const R omega = a.smag ? R(1) / smag_tau(ps, m, a.tau, a.cs) : a.omega;
   if (a.bounce == LBM_BOUNCE_HARD) {
-    if (a.flags[c] & 1) reverse_all(g);
+    if (fl & 1) reverse_all(g);
   } else if (a.bounce == LBM_BOUNCE_SOFT) {
     R cc[Q];
     sfor<Q>([&](auto I) {
       constexpr int i = I;
       cc[i] = s[i];
     });
-    if constexpr (COLL == LBM_COLL_MRT) collide_mrt(cc, omega);
-    else collide_bgk(cc, omega);
+    if constexpr (COLL == LBM_COLL_MRT) collide_mrt(cc, omega, ps, m);
+    else collide_bgk(cc, omega, m);
     const R gp = blend_vjp(cc, g, a.phi[c]);
-    if (a.phi_acc) a.phi_acc[c] += double(gp);
+    if (a.phi_acc && !skip) a.phi_acc[c] += double(gp);
   }
   if (pre_reset) {
     sfor<Q>([&](auto I) {
@@ -158,8 +168,8 @@ __device__ __forceinline__ bool collide_transpose(const AdjArgs<R>& a, R* s, R* 
       g[i] = R(0);
     });
   } else {
-    const R gom = (COLL == LBM_COLL_MRT) ? mrt_vjp(s, g, omega) : bgk_vjp(s, g, omega);
-    if (a.smag) smag_vjp(s, g, a.tau, a.cs, -gom * omega * omega);
+    const R gom = (COLL == LBM_COLL_MRT) ? mrt_vjp(ps, m, g, omega) : bgk_vjp(ps, m, g, omega);
+    if (a.smag) smag_vjp(ps, m, g, a.tau, a.cs, -gom * omega * omega);
   }
   return pre_reset;
 }
@@ -170,13 +180,19 @@ template <class S, int COLL, int MODE, bool LINK>
 __device__ __forceinline__ void adjoint_cell(const AdjArgs<typename S::R>& a, const typename S::T* __restrict__ P,
                                              const typename S::R* __restrict__ lam_in,
                                              typename S::R* __restrict__ lam_out, const double* __restrict__ gF,
-                                             uint32_t c, int b) {
+                                             uint32_t c, int b, uint32_t fl, const typename S::R* qv) {
   using R = typename S::R;
-  const size_t N = a.N;
+  const size_t N = a.P;
+  // a dense-sweep thread on a link cell computes but writes nothing (the
+  // cell belongs to a link workgroup)
+  const bool skip = !LINK && (fl & 2u);
   int x, y, z;
   adj_coords(a, c, x, y, z);
-  R s[Q], own[LINK ? Q : 1];
-  pull_cell<S, LINK>(a, P, c, x, y, z, b, s, own);
+  R s[Q], own[LINK ? Q : 1], v2[LINK ? Q : 1];
+  pull_cell<S, LINK>(a, P, c, x, y, z, b, s, own, qv, v2);
+  // the inlet's density depends on the pulled values it keeps
+  R rho_in = R(0);
+  if (x == 0) rho_in = zou_he_in_rho(s, a.uin[0]);
   apply_zou_he(a, s, x);
   const bool post_reset = (a.guard == LBM_GUARD_POST) && guard_bad(s);
   if (post_reset) set_weights(s);
@@ -185,7 +201,7 @@ __device__ __forceinline__ void adjoint_cell(const AdjArgs<typename S::R>& a, co
     constexpr int i = I;
     g[i] = lam_in[i * N + c];
   });
-  if constexpr (MODE == LBM_MODE_FULL) collide_transpose<R, COLL>(a, s, g, c);
+  if constexpr (MODE == LBM_MODE_FULL) collide_transpose<R, COLL>(a, s, g, c, fl, skip);
   // momentum-exchange force uses S_{t+1}(x)_opp(j) for links j (forces.py:95-118)
   if constexpr (LINK) {
     if (gF) {
@@ -193,7 +209,7 @@ __device__ __forceinline__ void adjoint_cell(const AdjArgs<typename S::R>& a, co
       sfor<Q>([&](auto J) {
         constexpr int j = J;
         if constexpr (j != 0) {
-          if (a.link_q[size_t(j - 1) * a.n_links + b] >= 0.0) {
+          if (qv[j] >= R(0)) {
             R v = R(0);
             add_scaled<EX[j]>(v, gx);
             add_scaled<EY[j]>(v, gy);
@@ -210,14 +226,12 @@ __device__ __forceinline__ void adjoint_cell(const AdjArgs<typename S::R>& a, co
       g[i] = R(0);
     });
   }
-  // Zou-He^T (the outlet was applied last); the inlet needs the pulled values
+  // Zou-He^T (the outlet was applied last)
   if (x == a.nx - 1) zou_he_out_vjp(g, a.rho_out);
   if (x == 0) {
-    R sp[Q], tmp[LINK ? Q : 1];
-    pull_cell<S, LINK>(a, P, c, x, y, z, b, sp, tmp);
     double gu[3] = {0.0, 0.0, 0.0};
-    zou_he_in_vjp(sp, g, a.uin[0], a.uin[1], a.uin[2], gu);
-    if (a.u_acc) {
+    zou_he_in_vjp(rho_in, g, a.uin[0], a.uin[1], a.uin[2], gu);
+    if (a.u_acc && !skip) {
       const size_t row = size_t(c) / size_t(a.nx);
       a.u_acc[3 * row] += gu[0];
       a.u_acc[3 * row + 1] += gu[1];
@@ -225,6 +239,7 @@ __device__ __forceinline__ void adjoint_cell(const AdjArgs<typename S::R>& a, co
     }
   }
   // owner writes and Bouzidi bookkeeping
+  if (skip) return;
   lam_out[c] = g[0];
   sfor<Q>([&](auto I) {
     constexpr int i = I;
@@ -233,7 +248,7 @@ __device__ __forceinline__ void adjoint_cell(const AdjArgs<typename S::R>& a, co
       bool link = false;
       R qo = R(-1);
       if constexpr (LINK) {
-        qo = R(a.link_q[size_t(io - 1) * a.n_links + b]);
+        qo = qv[io];
         link = qo >= R(0);
       }
       const bool normal = !link && !x_fallback<i>(x, a.nx);
@@ -246,8 +261,7 @@ __device__ __forceinline__ void adjoint_cell(const AdjArgs<typename S::R>& a, co
           if (qo >= R(0.5)) {
             if (qo <= R(1)) dsdq = -(own[io] - own[i]) / (R(2) * qo * qo);
           } else {
-            const R cval = S::ld_c(P, io * N + next_cell<i>(a, x, y, z), a.off[io]);
-            dsdq = R(2) * (own[io] - cval);
+            dsdq = R(2) * (own[io] - v2[i]);
           }
           a.dq_acc[size_t(io - 1) * a.n_links + b] += double(g[i]) * double(dsdq);
         }
@@ -256,23 +270,47 @@ __device__ __forceinline__ void adjoint_cell(const AdjArgs<typename S::R>& a, co
   });
 }
 
+// Link cells (Bouzidi transpose, mu_link, dq): a small launch ahead of the
+// dense sweep, as in the forward step.
 template <class S, int COLL, int MODE>
-__global__ __launch_bounds__(kAdjBlock) void k_adjoint(AdjArgs<typename S::R> a, const typename S::T* __restrict__ P,
+__global__ __launch_bounds__(kAdjLinkBlock) void k_adjoint_links(AdjArgs<typename S::R> a,
+                                                                 const typename S::T* __restrict__ P,
+                                                                 const typename S::R* __restrict__ lam_in,
+                                                                 typename S::R* __restrict__ lam_out,
+                                                                 const double* __restrict__ gF) {
+  using R = typename S::R;
+  const int b = int(blockIdx.x) * kAdjLinkBlock + int(threadIdx.x);
+  if (b >= a.n_links) return;
+  // the cell index and its q values in one round trip
+  const uint32_t c = uint32_t(a.link_cell[b]);
+  R qv[Q];
+  qv[0] = R(-1);
+  sfor<Q>([&](auto I) {
+    constexpr int i = I;
+    if constexpr (i != 0) qv[i] = R(a.link_q[size_t(i - 1) * a.n_links + b]);
+  });
+  adjoint_cell<S, COLL, MODE, true>(a, P, lam_in, lam_out, gF, c, b, a.flags[c], qv);
+}
+
+// Occupancy target of the dense adjoint sweep: the FP32 MRT transpose fits
+// 128 VGPRs (4 waves per SIMD) with a few spills confined to the inlet plane;
+// the other variants keep the compiler's allocation.
+template <class S, int COLL>
+constexpr int adj_waves_per_eu() {
+  return (sizeof(typename S::R) == 4 && COLL == LBM_COLL_MRT) ? 4 : 1;
+}
+
+// Dense sweep; link cells are recomputed but written by k_adjoint_links.
+template <class S, int COLL, int MODE>
+__global__ __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(adj_waves_per_eu<S, COLL>()))) void k_adjoint(AdjArgs<typename S::R> a, const typename S::T* __restrict__ P,
                                                        const typename S::R* __restrict__ lam_in,
                                                        typename S::R* __restrict__ lam_out,
                                                        const double* __restrict__ gF) {
-  const uint32_t bid = blockIdx.x;
-  if (int(bid) < a.nlb) {
-    const int b = int(bid) * kAdjBlock + int(threadIdx.x);
-    if (b >= a.n_links) return;
-    adjoint_cell<S, COLL, MODE, true>(a, P, lam_in, lam_out, gF, uint32_t(a.link_cell[b]), b);
-    return;
-  }
-  const uint32_t tile = xcd_swizzle(bid - uint32_t(a.nlb), uint32_t(a.ndb));
+  const uint32_t tile = xcd_swizzle(blockIdx.x, uint32_t(a.ndb));
   const uint32_t c = tile * kAdjBlock + threadIdx.x;
   if (c >= a.N) return;
-  if (a.flags && (a.flags[c] & 2)) return;
-  adjoint_cell<S, COLL, MODE, false>(a, P, lam_in, lam_out, gF, c, -1);
+  adjoint_cell<S, COLL, MODE, false>(a, P, lam_in, lam_out, gF, c, -1, a.flags ? uint32_t(a.flags[c]) : 0u,
+                                     nullptr);
 }
 
 // Prologue transpose: dL/dS_0 = ([pre guard] collide bounce)^T lambda_0, cell-local.
@@ -284,14 +322,14 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjoint_collide(AdjArgs<typename 
   using R = typename S::R;
   const uint32_t c = blockIdx.x * kAdjBlock + threadIdx.x;
   if (c >= a.N) return;
-  const size_t N = a.N;
+  const size_t N = a.P;
   R s[Q], g[Q];
   sfor<Q>([&](auto I) {
     constexpr int i = I;
     s[i] = S::ld_c(S0, i * N + c, a.off[i]);
     g[i] = lam_in[i * N + c];
   });
-  collide_transpose<R, COLL>(a, s, g, c);
+  collide_transpose<R, COLL>(a, s, g, c, a.flags ? uint32_t(a.flags[c]) : 0u, false);
   sfor<Q>([&](auto I) {
     constexpr int i = I;
     g_out[i * N + c] = g[i];
@@ -328,8 +366,10 @@ template <typename R>
 int make_adj_args(const lbm_step_desc* d, const lbm_adjoint_desc* ad, int mode, AdjArgs<R>& a) {
   if (!d || !ad || d->nx < 2 || d->ny <= 0 || d->nz <= 0) return LBM_E_ARG;
   const long long n = 1LL * d->nx * d->ny * d->nz;
-  if (n * Q >= (1LL << 31)) return LBM_E_ARG;
+  const long long pstride = d->pop_stride ? d->pop_stride : n;
+  if (pstride < n || pstride * Q >= (1LL << 31)) return LBM_E_ARG;
   if (!d->periodic_yz) return LBM_E_ARG;  // the owner-write scheme needs periodic y/z
+  a.P = uint32_t(pstride);
   a.nx = d->nx;
   a.ny = d->ny;
   a.nz = d->nz;
@@ -341,6 +381,7 @@ int make_adj_args(const lbm_step_desc* d, const lbm_adjoint_desc* ad, int mode, 
   a.guard = d->guard;
   a.n_links = d->n_links;
   a.tau = R(d->tau);
+  a.omega = R(1) / a.tau;
   a.cs = R(d->smag_cs);
   for (int k = 0; k < 3; ++k) {
     a.uin[k] = R(d->u_in[k]);
@@ -359,7 +400,7 @@ int make_adj_args(const lbm_step_desc* d, const lbm_adjoint_desc* ad, int mode, 
   if (a.bounce == LBM_BOUNCE_HARD && !a.flags) return LBM_E_ARG;
   if (a.bounce == LBM_BOUNCE_SOFT && !a.phi) return LBM_E_ARG;
   if (a.n_links < 0 || (a.n_links > 0 && (!a.link_cell || !a.link_q || !a.flags || !a.mu_link))) return LBM_E_ARG;
-  a.nlb = (mode == LBM_MODE_COLLIDE) ? 0 : (a.n_links + kAdjBlock - 1) / kAdjBlock;
+  a.nlb = (mode == LBM_MODE_COLLIDE) ? 0 : (a.n_links + kAdjLinkBlock - 1) / kAdjLinkBlock;
   a.ndb = int((n + kAdjBlock - 1) / kAdjBlock);
   return LBM_OK;
 }
@@ -381,8 +422,22 @@ int launch_adjoint(const lbm_step_desc* d, const lbm_adjoint_desc* ad, int mode,
     if (mrt) hipLaunchKernelGGL((k_adjoint_collide<S, LBM_COLL_MRT>), grid, kAdjBlock, 0, st, a, P, li, lo);
     else hipLaunchKernelGGL((k_adjoint_collide<S, LBM_COLL_BGK>), grid, kAdjBlock, 0, st, a, P, li, lo);
   } else {
-    const unsigned grid = unsigned(a.nlb + a.ndb);
-#define LBM_ADJ(COLL, MODE) hipLaunchKernelGGL((k_adjoint<S, COLL, MODE>), grid, kAdjBlock, 0, st, a, P, li, lo, gF)
+    const unsigned grid = unsigned(a.ndb);
+#define LBM_ADJ(COLL, MODE)                                                                                     \
+  do {                                                                                                          \
+    auto dense = [&](hipStream_t q) {                                                                           \
+      hipLaunchKernelGGL((k_adjoint<S, COLL, MODE>), grid, kAdjBlock, 0, q, a, P, li, lo, gF);                  \
+    };                                                                                                          \
+    if (a.nlb > 0) {                                                                                            \
+      auto links = [&](hipStream_t q) {                                                                         \
+        hipLaunchKernelGGL((k_adjoint_links<S, COLL, MODE>), unsigned(a.nlb), kAdjLinkBlock, 0, q, a, P, li, lo, gF); \
+      };                                                                                                        \
+      links(st);                                                                                                \
+      dense(st);                                                                                                \
+    } else {                                                                                                    \
+      dense(st);                                                                                                \
+    }                                                                                                           \
+  } while (0)
     if (mode == LBM_MODE_FULL) {
       if (mrt) LBM_ADJ(LBM_COLL_MRT, LBM_MODE_FULL);
       else LBM_ADJ(LBM_COLL_BGK, LBM_MODE_FULL);

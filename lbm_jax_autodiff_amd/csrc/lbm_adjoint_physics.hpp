@@ -33,24 +33,36 @@ __device__ __forceinline__ void add_macro_pullback(R* g, const Macro<R>& m, R Gr
 // Cotangents of (rho, u) from a cotangent h on f_eq(rho, u):
 //   d feq_k / d rho|u = w_k (1 + 3 e.u + 4.5 (e.u)^2 - 1.5 u^2),
 //   d feq_k / d u_a   = w_k rho (3 e_ka + 9 (e_k.u) e_ka - 3 u_a).
-template <typename R>
-__device__ __forceinline__ void feq_vjp(const R* h, const Macro<R>& m, R& Grho, R& Gux, R& Guy, R& Guz) {
-  Grho = R(0);
+// h is given as its rest value and its pair sums hs / differences hd
+// (hd is all zero when ODD is false); per pair the two directions give
+//   w [hs (1 + 4.5 eu^2 - 1.5 u^2) + 3 eu hd]  and  w e_a (3 hd + 9 eu hs).
+template <bool ODD, typename R>
+__device__ __forceinline__ void feq_vjp(R h0, const R* hs, const R* hd, const Macro<R>& m, R& Grho, R& Gux,
+                                        R& Guy, R& Guz) {
+  constexpr float w0 = weight(0);
+  Grho = h0 * R(w0) * (R(1) - R(1.5) * m.usq);
+  R hw = h0 * R(w0);
   Gux = R(0);
   Guy = R(0);
   Guz = R(0);
-  R hw = R(0);  // sum_k h_k w_k
-  sfor<Q>([&](auto K) {
-    constexpr int k = K;
-    constexpr float wk = weight(k);
-    const R eu = edot<k>(m.ux, m.uy, m.uz);
-    const R hk = h[k] * R(wk);
-    Grho += hk * (R(1) + R(3) * eu + R(4.5) * eu * eu - R(1.5) * m.usq);
-    hw += hk;
-    const R t = hk * (R(3) + R(9) * eu);
-    add_scaled<EX[k]>(Gux, t);
-    add_scaled<EY[k]>(Guy, t);
-    add_scaled<EZ[k]>(Guz, t);
+  sfor<NP>([&](auto P) {
+    constexpr int k = P;
+    constexpr int a = PA[k];
+    constexpr float w = weight(a);
+    const R eu = edot<a>(m.ux, m.uy, m.uz);
+    const R ws = R(w) * hs[k];
+    R gr = ws * (R(1) + R(4.5) * eu * eu - R(1.5) * m.usq);
+    R t = R(9) * eu * ws;
+    if constexpr (ODD) {
+      const R wd = R(w) * hd[k];
+      gr += R(3) * eu * wd;
+      t += R(3) * wd;
+    }
+    Grho += gr;
+    hw += ws;
+    add_scaled<EX[a]>(Gux, t);
+    add_scaled<EY[a]>(Guy, t);
+    add_scaled<EZ[a]>(Guz, t);
   });
   Gux = m.rho * (Gux - R(3) * m.ux * hw);
   Guy = m.rho * (Guy - R(3) * m.uy * hw);
@@ -58,21 +70,29 @@ __device__ __forceinline__ void feq_vjp(const R* h, const Macro<R>& m, R& Grho, 
 }
 
 // BGK: c = s - omega (s - feq(s)).  g <- J^T g; returns dL/domega.
+// J^T g = (1 - omega) g + omega (d feq / d s)^T g.
 template <typename R>
-__device__ __forceinline__ R bgk_vjp(const R* s, R* g, R omega) {
-  const Macro<R> m = macro(s);
-  R h[Q];
-  R gom = R(0);
+__device__ __forceinline__ R bgk_vjp(const Pairs<R>& ps, const Macro<R>& m, R* g, R omega) {
+  const Pairs<R> pg = to_pairs(g);
+  // dL/domega = -sum_i g_i (s_i - feq_i), per pair
+  //   = -(gs (ss - 2 sym) + gd (sd - 2 asym)) / 2
+  R gom = -pg.f0 * (ps.f0 - R(weight(0)) * m.rho * (R(1) - R(1.5) * m.usq));
+  R acc = R(0);
+  sfor<NP>([&](auto P) {
+    constexpr int k = P;
+    R sym, asym;
+    feq_pair<k>(m, sym, asym);
+    acc += pg.s[k] * (ps.s[k] - R(2) * sym) + pg.d[k] * (ps.d[k] - R(2) * asym);
+  });
+  gom -= R(0.5) * acc;
+  R Gr, Gx, Gy, Gz;
+  feq_vjp<true>(pg.f0, pg.s, pg.d, m, Gr, Gx, Gy, Gz);
+  const R k1 = R(1) - omega;
   sfor<Q>([&](auto I) {
     constexpr int i = I;
-    const R e = feq<i>(m.rho, m.ux, m.uy, m.uz, m.usq);
-    gom -= g[i] * (s[i] - e);
-    h[i] = omega * g[i];
-    g[i] = (R(1) - omega) * g[i];
+    g[i] *= k1;
   });
-  R Gr, Gx, Gy, Gz;
-  feq_vjp(h, m, Gr, Gx, Gy, Gz);
-  add_macro_pullback(g, m, Gr, Gx, Gy, Gz);
+  add_macro_pullback(g, m, omega * Gr, omega * Gx, omega * Gy, omega * Gz);
   return gom;
 }
 
@@ -98,8 +118,8 @@ __device__ __forceinline__ void meq_vjp(const R* z, const Macro<R>& m, R& Grho, 
 // MRT: c = s - M^T D (M s - meq(s)), D = diag(rate_k / ||M_k||^2).
 // J^T g = g - M^T h + (d meq/d s)^T h with h = D M g.  Returns dL/domega.
 template <typename R>
-__device__ __forceinline__ R mrt_vjp(const R* s, R* g, R omega) {
-  const Macro<R> mac = macro(s);
+__device__ __forceinline__ R mrt_vjp(const Pairs<R>& ps, const Macro<R>& mac, R* g, R omega) {
+  const Pairs<R> pg = to_pairs(g);
   R h[Q];
   R gom = R(0);
   sfor<Q>([&](auto K) {
@@ -108,9 +128,9 @@ __device__ __forceinline__ R mrt_vjp(const R* s, R* g, R omega) {
       h[k] = R(0);
     } else {
       constexpr double inv_n = 1.0 / double(mrt_norm(k));
-      const R Mg = moment<k>(g);
+      const R Mg = moment<k>(pg);
       if constexpr (mrt_is_stress(k)) {
-        const R dm = moment<k>(s) - meq<k>(mac.rho, mac.ux, mac.uy, mac.uz, mac.usq);
+        const R dm = moment<k>(ps) - meq<k>(mac.rho, mac.ux, mac.uy, mac.uz, mac.usq);
         gom -= Mg * dm * R(inv_n);
         h[k] = omega * Mg * R(inv_n);
       } else {
@@ -127,10 +147,11 @@ __device__ __forceinline__ R mrt_vjp(const R* s, R* g, R omega) {
 
 // Smagorinsky: tau_eff(s) = (tau + sqrt(tau^2 + 18 cs^2 |Pi| / (rho + 1e-10))) / 2,
 // Pi_ab = sum_i e_ia e_ib (s_i - feq_i(s)).  Adds g_teff * d tau_eff / d s to g.
+// The cotangent of the non-equilibrium part is even in e (the same for a
+// and opp(a)) and zero at rest.
 template <typename R>
-__device__ __forceinline__ void smag_vjp(const R* s, R* g, R tau, R cs, R g_teff) {
-  const Macro<R> m = macro(s);
-  const Stress<R> st = neq_stress(s, m);
+__device__ __forceinline__ void smag_vjp(const Pairs<R>& ps, const Macro<R>& m, R* g, R tau, R cs, R g_teff) {
+  const Stress<R> st = neq_stress(ps, m);
   const R pi = sqrt(st.xx * st.xx + st.yy * st.yy + st.zz * st.zz +
                     R(2) * (st.xy * st.xy + st.xz * st.xz + st.yz * st.yz));
   const R den = m.rho + R(1e-10);
@@ -144,32 +165,39 @@ __device__ __forceinline__ void smag_vjp(const R* s, R* g, R tau, R cs, R g_teff
   const R inv = pi > R(0) ? gpi / pi : R(0);
   const R gxx = inv * st.xx, gyy = inv * st.yy, gzz = inv * st.zz;
   const R gxy = R(2) * inv * st.xy, gxz = R(2) * inv * st.xz, gyz = R(2) * inv * st.yz;
-  // cotangent of the non-equilibrium part n_i = s_i - feq_i
-  R hn[Q];
-  sfor<Q>([&](auto I) {
-    constexpr int i = I;
+  R hs[NP];
+  sfor<NP>([&](auto P) {
+    constexpr int k = P;
+    constexpr int a = PA[k], o = opp(PA[k]);
     R v = R(0);
-    add_scaled<EX[i] * EX[i]>(v, gxx);
-    add_scaled<EY[i] * EY[i]>(v, gyy);
-    add_scaled<EZ[i] * EZ[i]>(v, gzz);
-    add_scaled<EX[i] * EY[i]>(v, gxy);
-    add_scaled<EX[i] * EZ[i]>(v, gxz);
-    add_scaled<EY[i] * EZ[i]>(v, gyz);
-    hn[i] = v;
-    g[i] += v;
+    add_scaled<EX[a] * EX[a]>(v, gxx);
+    add_scaled<EY[a] * EY[a]>(v, gyy);
+    add_scaled<EZ[a] * EZ[a]>(v, gzz);
+    add_scaled<EX[a] * EY[a]>(v, gxy);
+    add_scaled<EX[a] * EZ[a]>(v, gxz);
+    add_scaled<EY[a] * EZ[a]>(v, gyz);
+    g[a] += v;
+    g[o] += v;
+    hs[k] = R(2) * v;
   });
   R Gr, Gx, Gy, Gz;
-  feq_vjp(hn, m, Gr, Gx, Gy, Gz);
+  feq_vjp<false>(R(0), hs, hs, m, Gr, Gx, Gy, Gz);
   add_macro_pullback(g, m, g_teff * drho - Gr, -Gx, -Gy, -Gz);
 }
 
-// Zou-He inlet transpose.  s holds the pulled (pre-inlet) values; g the
-// cotangent of the post-inlet values.  Accumulates dL/du_in into gu.
+// Density the Zou-He inlet computes from the pulled values it keeps
+// (boundary.py:80-124): rho = (K0 + 2 K-) / (1 - u_x).
 template <typename R>
-__device__ __forceinline__ void zou_he_in_vjp(const R* s, R* g, R ux, R uy, R uz, double* gu) {
+__device__ __forceinline__ R zou_he_in_rho(const R* s, R ux) {
   const R k0 = s[0] + s[3] + s[4] + s[5] + s[6] + s[15] + s[16] + s[17] + s[18];
   const R km = s[2] + s[8] + s[10] + s[12] + s[14];
-  const R rho = (k0 + R(2) * km) / (R(1) - ux);
+  return (k0 + R(2) * km) / (R(1) - ux);
+}
+
+// Zou-He inlet transpose.  rho = zou_he_in_rho of the pulled values; g the
+// cotangent of the post-inlet values.  Accumulates dL/du_in into gu.
+template <typename R>
+__device__ __forceinline__ void zou_he_in_vjp(R rho, R* g, R ux, R uy, R uz, double* gu) {
   const R g1 = g[1], g7 = g[7], g9 = g[9], g11 = g[11], g13 = g[13];
   const R grho = g1 * ux * R(1.0 / 3.0) + (g7 * (ux + uy) + g9 * (ux - uy) + g11 * (ux + uz) + g13 * (ux - uz)) *
                  R(1.0 / 6.0);

@@ -1,8 +1,16 @@
 // Cell-local D3Q19 physics shared by the per-op and the fused kernels.
+//
 // Everything works on 19 values held in registers.  Loops over the velocity
-// set and the moment rows use sfor, so every lattice constant (velocity
-// components, weights, opposites, integer moment-matrix entries) is a
+// set and the moment rows use sfor, so every lattice constant is a
 // compile-time constant and zero terms are never emitted.
+//
+// The kernels are arithmetic-bound on gfx950 (packed FP32 is not used), so the
+// collision works on opposite pairs (a, opp(a)): with s = f_a + f_opp and
+// d = f_a - f_opp, the moments that are even in the lattice velocity (rho, e,
+// eps, the stresses) depend on the nine sums only and the odd ones (j, q, the
+// ghost m) on the nine differences only; the transposed transform splits the
+// same way.  This halves the work of the moment transform against a dense
+// 19 x 19 product and is exact algebra (float rounding differs by ulps).
 #pragma once
 
 #include "lbm_common.hpp"
@@ -23,24 +31,74 @@ __device__ __forceinline__ void add_scaled(R& acc, R v) {
   else if constexpr (C != 0) acc += R(C) * v;
 }
 
-// rho = sum f_i; u = sum f_i e_i / rho  (equilibrium.py:22-87)
+// The nine pair representatives a (opp(a) > a).
+constexpr int NP = 9;
+constexpr int PA[NP] = {1, 3, 5, 7, 8, 11, 12, 15, 16};
+
+__host__ __device__ constexpr bool row_even(int k) {
+  return k == 0 || k == 1 || k == 2 || (k >= 9 && k <= 15);
+}
+
+// the parity property the pair form relies on, checked at compile time
+__host__ __device__ constexpr bool mrt_parity_holds() {
+  for (int k = 0; k < Q; ++k)
+    for (int p = 0; p < NP; ++p) {
+      const int a = PA[p], o = opp(PA[p]);
+      if (row_even(k) && mrt_m(k, a) != mrt_m(k, o)) return false;
+      if (!row_even(k) && (mrt_m(k, a) != -mrt_m(k, o) || mrt_m(k, 0) != 0)) return false;
+    }
+  for (int p = 0; p < NP; ++p)
+    if (opp(PA[p]) <= PA[p]) return false;
+  return true;
+}
+static_assert(mrt_parity_holds(), "D3Q19 moment rows must be even or odd under e -> -e");
+
 template <typename R>
-__device__ __forceinline__ Macro<R> macro(const R* f) {
-  R rho = R(0), jx = R(0), jy = R(0), jz = R(0);
-  sfor<Q>([&](auto I) {
-    constexpr int i = I;
-    rho += f[i];
-    add_scaled<EX[i]>(jx, f[i]);
-    add_scaled<EY[i]>(jy, f[i]);
-    add_scaled<EZ[i]>(jz, f[i]);
+struct Pairs {
+  R f0;
+  R s[NP];  // f_a + f_opp(a)
+  R d[NP];  // f_a - f_opp(a)
+};
+
+template <typename R>
+__device__ __forceinline__ Pairs<R> to_pairs(const R* f) {
+  Pairs<R> p;
+  p.f0 = f[0];
+  sfor<NP>([&](auto P) {
+    constexpr int k = P;
+    constexpr int a = PA[k], o = opp(PA[k]);
+    p.s[k] = f[a] + f[o];
+    p.d[k] = f[a] - f[o];
+  });
+  return p;
+}
+
+// rho = sum f; u = sum f e / rho (equilibrium.py:22-87), from the pairs:
+// opposite directions contribute e_a (f_a - f_opp) to the momentum.
+template <typename R>
+__device__ __forceinline__ Macro<R> macro(const Pairs<R>& p) {
+  R rho = p.f0, jx = R(0), jy = R(0), jz = R(0);
+  sfor<NP>([&](auto P) {
+    constexpr int k = P;
+    constexpr int a = PA[k];
+    rho += p.s[k];
+    add_scaled<EX[a]>(jx, p.d[k]);
+    add_scaled<EY[a]>(jy, p.d[k]);
+    add_scaled<EZ[a]>(jz, p.d[k]);
   });
   Macro<R> m;
   m.rho = rho;
-  m.ux = jx / rho;
-  m.uy = jy / rho;
-  m.uz = jz / rho;
+  const R inv = R(1) / rho;
+  m.ux = jx * inv;
+  m.uy = jy * inv;
+  m.uz = jz * inv;
   m.usq = m.ux * m.ux + m.uy * m.uy + m.uz * m.uz;
   return m;
+}
+
+template <typename R>
+__device__ __forceinline__ Macro<R> macro(const R* f) {
+  return macro(to_pairs(f));
 }
 
 // e_i . u for a compile-time direction
@@ -62,15 +120,35 @@ __device__ __forceinline__ R feq(R rho, R ux, R uy, R uz, R usq) {
   return R(w) * rho * (R(1) + R(3) * eu + R(4.5) * eu * eu - R(1.5) * usq);
 }
 
-// BGK relaxation (collision.py:89-100)
+// The even and odd parts of f_eq for pair k: f_eq_a = sym + asym,
+// f_eq_opp = sym - asym.
+template <int K, typename R>
+__device__ __forceinline__ void feq_pair(const Macro<R>& m, R& sym, R& asym) {
+  constexpr int a = PA[K];
+  constexpr float w = weight(a);
+  const R eu = edot<a>(m.ux, m.uy, m.uz);
+  const R wr = R(w) * m.rho;
+  sym = wr * (R(1) + R(4.5) * eu * eu - R(1.5) * m.usq);
+  asym = R(3) * wr * eu;
+}
+
+// BGK relaxation (collision.py:89-100) given the cell's macroscopic state.
+template <typename R>
+__device__ __forceinline__ void collide_bgk(R* f, R omega, const Macro<R>& m) {
+  f[0] = f[0] - omega * (f[0] - R(weight(0)) * m.rho * (R(1) - R(1.5) * m.usq));
+  sfor<NP>([&](auto P) {
+    constexpr int k = P;
+    constexpr int a = PA[k], o = opp(PA[k]);
+    R sym, asym;
+    feq_pair<k>(m, sym, asym);
+    f[a] = f[a] - omega * (f[a] - (sym + asym));
+    f[o] = f[o] - omega * (f[o] - (sym - asym));
+  });
+}
+
 template <typename R>
 __device__ __forceinline__ void collide_bgk(R* f, R omega) {
-  const Macro<R> m = macro(f);
-  sfor<Q>([&](auto I) {
-    constexpr int i = I;
-    const R e = feq<i>(m.rho, m.ux, m.uy, m.uz, m.usq);
-    f[i] = f[i] - omega * (f[i] - e);
-  });
+  collide_bgk(f, omega, macro(f));
 }
 
 // Equilibrium moments (lattice.py:342-391) for the non-conserved rows.
@@ -91,28 +169,53 @@ __device__ __forceinline__ R meq(R rho, R ux, R uy, R uz, R usq) {
   else return R(0);
 }
 
-// Moment K of f: sum_i M_Ki f_i (integer coefficients folded at compile time).
+// Moment K from the pairs: even rows use f0 and the sums, odd rows the differences.
 template <int K, typename R>
-__device__ __forceinline__ R moment(const R* f) {
+__device__ __forceinline__ R moment(const Pairs<R>& p) {
   R acc = R(0);
-  sfor<Q>([&](auto I) {
-    constexpr int i = I;
-    add_scaled<mrt_m(K, i)>(acc, f[i]);
-  });
+  if constexpr (row_even(K)) {
+    add_scaled<mrt_m(K, 0)>(acc, p.f0);
+    sfor<NP>([&](auto P) {
+      constexpr int k = P;
+      add_scaled<mrt_m(K, PA[k])>(acc, p.s[k]);
+    });
+  } else {
+    sfor<NP>([&](auto P) {
+      constexpr int k = P;
+      add_scaled<mrt_m(K, PA[k])>(acc, p.d[k]);
+    });
+  }
   return acc;
 }
 
-// acc_i -= sum_k M_ki d_k over the non-conserved rows (the transposed transform)
+template <int K, typename R>
+__device__ __forceinline__ R moment(const R* f) {
+  return moment<K>(to_pairs(f));
+}
+
+// f_i -= (M^T h)_i over the non-conserved rows, split into the even part (the
+// same for a and opp(a)) and the odd part (opposite signs).
 template <typename R>
-__device__ __forceinline__ void sub_transposed(R* f, const R* d) {
-  sfor<Q>([&](auto I) {
-    constexpr int i = I;
-    R acc = R(0);
+__device__ __forceinline__ void sub_transposed(R* f, const R* h) {
+  R e0 = R(0);
+  sfor<Q>([&](auto K) {
+    constexpr int k = K;
+    if constexpr (!mrt_is_conserved(k) && row_even(k)) add_scaled<mrt_m(k, 0)>(e0, h[k]);
+  });
+  f[0] -= e0;
+  sfor<NP>([&](auto P) {
+    constexpr int p = P;
+    constexpr int a = PA[p], o = opp(PA[p]);
+    R ev = R(0), od = R(0);
     sfor<Q>([&](auto K) {
       constexpr int k = K;
-      if constexpr (!mrt_is_conserved(k)) add_scaled<mrt_m(k, i)>(acc, d[k]);
+      if constexpr (!mrt_is_conserved(k)) {
+        if constexpr (row_even(k)) add_scaled<mrt_m(k, a)>(ev, h[k]);
+        else add_scaled<mrt_m(k, a)>(od, h[k]);
+      }
     });
-    f[i] -= acc;
+    f[a] -= ev + od;
+    f[o] -= ev - od;
   });
 }
 
@@ -121,42 +224,49 @@ __device__ __forceinline__ void sub_transposed(R* f, const R* d) {
 // which equals the reference's M^{-1}(m - S(m - m_eq)) since M^{-1} M = I.
 // Conserved rows have rate 0 and drop out.
 template <typename R>
-__device__ __forceinline__ void collide_mrt(R* f, R omega) {
-  const Macro<R> mac = macro(f);
-  R d[Q];
+__device__ __forceinline__ void collide_mrt(R* f, R omega, const Pairs<R>& pr, const Macro<R>& mac) {
+  R h[Q];
   sfor<Q>([&](auto K) {
     constexpr int k = K;
     if constexpr (mrt_is_conserved(k)) {
-      d[k] = R(0);
+      h[k] = R(0);
     } else {
       constexpr double inv_n = 1.0 / double(mrt_norm(k));
       const R rate = mrt_is_stress(k) ? omega : R(mrt_rate_fixed(k));
-      d[k] = rate * R(inv_n) * (moment<k>(f) - meq<k>(mac.rho, mac.ux, mac.uy, mac.uz, mac.usq));
+      h[k] = rate * R(inv_n) * (moment<k>(pr) - meq<k>(mac.rho, mac.ux, mac.uy, mac.uz, mac.usq));
     }
   });
-  sub_transposed(f, d);
+  sub_transposed(f, h);
 }
 
-// Non-equilibrium stress Pi_ab = sum_i e_ia e_ib (f_i - f_eq_i).
+template <typename R>
+__device__ __forceinline__ void collide_mrt(R* f, R omega) {
+  const Pairs<R> pr = to_pairs(f);
+  collide_mrt(f, omega, pr, macro(pr));
+}
+
+// Non-equilibrium stress Pi_ab = sum_i e_ia e_ib (f_i - f_eq_i): every
+// e_a e_b is even, so only the pair sums and the even part of f_eq enter.
 template <typename R>
 struct Stress {
   R xx, yy, zz, xy, xz, yz;
 };
 
 template <typename R>
-__device__ __forceinline__ Stress<R> neq_stress(const R* f, const Macro<R>& m) {
+__device__ __forceinline__ Stress<R> neq_stress(const Pairs<R>& p, const Macro<R>& m) {
   Stress<R> s{R(0), R(0), R(0), R(0), R(0), R(0)};
-  sfor<Q>([&](auto I) {
-    constexpr int i = I;
-    if constexpr (i != 0) {
-      const R n = f[i] - feq<i>(m.rho, m.ux, m.uy, m.uz, m.usq);
-      add_scaled<EX[i] * EX[i]>(s.xx, n);
-      add_scaled<EY[i] * EY[i]>(s.yy, n);
-      add_scaled<EZ[i] * EZ[i]>(s.zz, n);
-      add_scaled<EX[i] * EY[i]>(s.xy, n);
-      add_scaled<EX[i] * EZ[i]>(s.xz, n);
-      add_scaled<EY[i] * EZ[i]>(s.yz, n);
-    }
+  sfor<NP>([&](auto P) {
+    constexpr int k = P;
+    constexpr int a = PA[k];
+    R sym, asym;
+    feq_pair<k>(m, sym, asym);
+    const R n = p.s[k] - R(2) * sym;
+    add_scaled<EX[a] * EX[a]>(s.xx, n);
+    add_scaled<EY[a] * EY[a]>(s.yy, n);
+    add_scaled<EZ[a] * EZ[a]>(s.zz, n);
+    add_scaled<EX[a] * EY[a]>(s.xy, n);
+    add_scaled<EX[a] * EZ[a]>(s.xz, n);
+    add_scaled<EY[a] * EZ[a]>(s.yz, n);
   });
   return s;
 }
@@ -164,11 +274,16 @@ __device__ __forceinline__ Stress<R> neq_stress(const R* f, const Macro<R>& m) {
 // tau_eff = (tau + sqrt(tau^2 + 18 cs^2 |Pi| / (rho + 1e-10))) / 2
 // (collision.py:241-270; |Pi| is the Frobenius norm of the full tensor).
 template <typename R>
-__device__ __forceinline__ R smag_tau(const R* f, R tau, R cs) {
-  const Macro<R> m = macro(f);
-  const Stress<R> s = neq_stress(f, m);
+__device__ __forceinline__ R smag_tau(const Pairs<R>& p, const Macro<R>& m, R tau, R cs) {
+  const Stress<R> s = neq_stress(p, m);
   const R pi = sqrt(s.xx * s.xx + s.yy * s.yy + s.zz * s.zz + R(2) * (s.xy * s.xy + s.xz * s.xz + s.yz * s.yz));
   return R(0.5) * (tau + sqrt(tau * tau + R(18) * cs * cs * pi / (m.rho + R(1e-10))));
+}
+
+template <typename R>
+__device__ __forceinline__ R smag_tau(const R* f, R tau, R cs) {
+  const Pairs<R> p = to_pairs(f);
+  return smag_tau(p, macro(p), tau, cs);
 }
 
 // Zou-He velocity inlet at x = 0 (boundary.py:80-124): unknowns
@@ -200,40 +315,36 @@ __device__ __forceinline__ void zou_he_out(R* f, R rho_out) {
 
 // Stability guard predicate (state.py:158-160; examples/10:71-74).
 template <typename R>
-__device__ __forceinline__ bool guard_bad(const R* f) {
-  const Macro<R> m = macro(f);
+__device__ __forceinline__ bool guard_bad(const Macro<R>& m) {
   const R speed = sqrt(m.usq);
   return (m.rho < R(0.3)) | (m.rho > R(2.0)) | (speed > R(0.4)) | (m.rho != m.rho);
+}
+
+template <typename R>
+__device__ __forceinline__ bool guard_bad(const R* f) {
+  return guard_bad(macro(f));
 }
 
 // f <- f[opp] (hard solid bounce, state.py:146-149)
 template <typename R>
 __device__ __forceinline__ void reverse_all(R* f) {
-  sfor<Q>([&](auto I) {
-    constexpr int i = I;
-    constexpr int o = opp(i);
-    if constexpr (o > i) {
-      const R t = f[i];
-      f[i] = f[o];
-      f[o] = t;
-    }
+  sfor<NP>([&](auto P) {
+    constexpr int a = PA[P], o = opp(PA[P]);
+    const R t = f[a];
+    f[a] = f[o];
+    f[o] = t;
   });
 }
 
 // f <- (1 - p) f + p f[opp] (soft bounce, smooth.py:151-155)
 template <typename R>
 __device__ __forceinline__ void blend_opposite(R* f, R p) {
-  sfor<Q>([&](auto I) {
-    constexpr int i = I;
-    constexpr int o = opp(i);
-    if constexpr (o > i) {
-      const R a = f[i], b = f[o];
-      f[i] = (R(1) - p) * a + p * b;
-      f[o] = (R(1) - p) * b + p * a;
-    } else if constexpr (o == i) {
-      // rest population: (1 - p) f0 + p f0
-      f[i] = (R(1) - p) * f[i] + p * f[i];
-    }
+  f[0] = (R(1) - p) * f[0] + p * f[0];
+  sfor<NP>([&](auto P) {
+    constexpr int a = PA[P], o = opp(PA[P]);
+    const R x = f[a], y = f[o];
+    f[a] = (R(1) - p) * x + p * y;
+    f[o] = (R(1) - p) * y + p * x;
   });
 }
 
@@ -253,6 +364,28 @@ __device__ __forceinline__ void set_weights(R* f) {
     constexpr int i = I;
     f[i] = R(weight(i));
   });
+}
+
+// [pre guard reset] -> [Smagorinsky tau] -> BGK|MRT on one cell, sharing a
+// single pass over the pairs and the macroscopic fields between the guard,
+// the eddy viscosity and the collision (examples/10:71-78, state.py:133-145).
+// omega = 1/tau is passed in for the constant-tau case.  Returns the omega
+// used; sets `reset` when the guard fired.
+template <bool MRT, typename R>
+__device__ __forceinline__ R relax(R* f, R tau, R omega, bool pre_guard, R urx, R ury, R urz, bool smag, R cs,
+                                   bool& reset) {
+  Pairs<R> p = to_pairs(f);
+  Macro<R> m = macro(p);
+  reset = pre_guard && guard_bad(m);
+  if (reset) {
+    set_feq_unit(f, urx, ury, urz);
+    p = to_pairs(f);
+    m = macro(p);
+  }
+  if (smag) omega = R(1) / smag_tau(p, m, tau, cs);
+  if constexpr (MRT) collide_mrt(f, omega, p, m);
+  else collide_bgk(f, omega, m);
+  return omega;
 }
 
 }  // namespace lbm

@@ -101,14 +101,17 @@ class Lattice:
     ``storage`` f32 / f64 / f16 / bf16 (half formats store f_i - offset_i
     with float32 arithmetic); ``offset`` "weights" (w_i) or "inlet"
     (f_eq(1, inlet_velocity)) or 19 numbers.  ``cells_per_thread`` (1, 2
-    or 4; 0 = widest dividing nx) sets how many x-consecutive cells one thread
-    of the dense sweep updates with vector loads.
+    or 4 when it divides nx; 0 = 1, the fastest) sets how many x-consecutive
+    cells one thread of the dense sweep updates with vector loads.
+    ``pop_stride`` is the element distance between consecutive populations
+    of the state buffers (default: the library's padded recommendation;
+    N gives the dense (19, nz, ny, nx) layout).
     """
 
     def __init__(self, nx, ny, nz, *, tau, inlet_velocity=(0.05, 0.0, 0.0), rho_out=1.0, collision="bgk",
                  smagorinsky_cs=None, solid=None, porosity=None, q=None, guard="none", guard_velocity=None,
                  periodic_yz=True, storage="f32", offset="weights", device=None, dqdr=None,
-                 cells_per_thread=0):
+                 cells_per_thread=0, pop_stride=None):
         if storage not in _STORAGE:
             raise ValueError(f"storage must be one of {sorted(_STORAGE)}")
         if guard not in _GUARD:
@@ -180,20 +183,36 @@ class Lattice:
         d.link_cell = L.ptr(self.links.cell)
         d.link_q = L.ptr(self.links.q)
         d.cells_per_thread = int(cells_per_thread)
+        lib = L.load()
+        self.P = int(lib.lbm_state_stride(ctypes.byref(d))) if pop_stride is None else int(pop_stride)
+        if self.P < self.N or 19 * self.P >= 2 ** 31:
+            raise ValueError(f"pop_stride must be in [N, 2^31 / 19); got {self.P} for N = {self.N}")
+        d.pop_stride = self.P
         self.desc = d
-        self.slots = int(L.load().lbm_step_force_slots(ctypes.byref(d)))
+        self.slots = int(lib.lbm_step_force_slots(ctypes.byref(d)))
 
     # ---- state buffers ---------------------------------------------------
-    def new_state(self) -> torch.Tensor:
-        return torch.empty((19, self.nz, self.ny, self.nx), dtype=self.buf_dtype, device=self.device)
+    def new_state(self, dtype: Optional[torch.dtype] = None) -> torch.Tensor:
+        """Uninitialised (19, nz, ny, nx) buffer with population stride P
+        (storage dtype by default; cotangent buffers pass their real dtype)."""
+        flat = torch.empty(19 * self.P, dtype=dtype or self.buf_dtype, device=self.device)
+        return flat.as_strided((19, self.nz, self.ny, self.nx), (self.P, self.ny * self.nx, self.nx, 1))
+
+    def clone_state(self, buf: torch.Tensor) -> torch.Tensor:
+        out = self.new_state(buf.dtype)
+        out.copy_(buf)
+        return out
 
     def to_state(self, f: torch.Tensor) -> torch.Tensor:
         """(19, nx, ny, nz) float tensor -> state buffer in the storage format."""
         F.require_cuda(f, "f")
-        fs = F.pop_storage(f)
+        fs = f.permute(0, 3, 2, 1)
         if self.storage in ("f32", "f64"):
-            return fs.to(self.buf_dtype).clone() if fs.dtype != self.buf_dtype else fs.clone()
-        src = fs.to(torch.float32).contiguous()
+            out = self.new_state()
+            out.copy_(fs)
+            return out
+        src = self.new_state(torch.float32)
+        src.copy_(fs)
         out = self.new_state()
         L.call("lbm_convert_state", ctypes.byref(self.desc), L.LBM_F32, L.ptr(src), self.tag, L.ptr(out),
                L.stream_handle(self.device))
@@ -203,7 +222,7 @@ class Lattice:
         """State buffer -> float32/float64 tensor indexed (19, nx, ny, nz)."""
         if self.storage in ("f32", "f64"):
             return F.pop_view(buf)
-        out = torch.empty((19, self.nz, self.ny, self.nx), dtype=torch.float32, device=self.device)
+        out = self.new_state(torch.float32)
         L.call("lbm_convert_state", ctypes.byref(self.desc), self.tag, L.ptr(buf), L.LBM_F32, L.ptr(out),
                L.stream_handle(self.device))
         return F.pop_view(out)
@@ -254,11 +273,11 @@ class Lattice:
         cur, nxt = a, b
         for t in range(n_steps - 1):
             if keep is not None and keep_every and t % keep_every == 0:
-                keep.append((t, cur.clone()))
+                keep.append((t, self.clone_state(cur)))
             self.step(cur, nxt, None if fpart is None else fpart[t])
             cur, nxt = nxt, cur
         if keep is not None and keep_every and (n_steps - 1) % keep_every == 0:
-            keep.append((n_steps - 1, cur.clone()))
+            keep.append((n_steps - 1, self.clone_state(cur)))
         s_final = s0  # reuse the prologue input buffer for S_N
         self.stream(cur, s_final, None if fpart is None else fpart[n_steps - 1])
         F_hist = None

@@ -28,6 +28,9 @@ class _CountingLattice:
     def new_state(self):
         return torch.zeros(1)
 
+    def clone_state(self, buf):
+        return buf.clone()
+
     def step(self, src, dst):
         self.steps += 1
         dst.copy_(src + 1)
@@ -128,3 +131,20 @@ def test_scatter_plan_is_sorted_and_complete():
     n_hi = int((qs[1:, cells][qs[1:, cells] >= 0] >= 0.5).sum())
     assert int(src.numel()) == n_force + n_links + n_links  # own + (own i | neighbour) + force
     assert n_hi <= n_links
+
+
+def test_scatter_plan_uses_population_stride():
+    """Targets address population i of cell c at i * P + c for a padded stride P."""
+    nx, ny, nz = 6, 5, 4
+    q = random_q(nx, ny, nz, seed=5, p_wall=0.3)
+    N = nx * ny * nz
+    qs = q.permute(0, 3, 2, 1).contiguous().reshape(19, N)
+    cells = torch.nonzero((qs[1:] >= 0).any(0)).reshape(-1)
+    links = types.SimpleNamespace(n=int(cells.numel()), cell=cells.to(torch.int32), q=qs[1:, cells].clone())
+    dense = types.SimpleNamespace(links=links, nx=nx, ny=ny, nz=nz, N=N, device=torch.device("cpu"))
+    P = N + 37
+    padded = types.SimpleNamespace(links=links, nx=nx, ny=ny, nz=nz, N=N, P=P, device=torch.device("cpu"))
+    t0, s0, src0, c0 = A.scatter_plan(dense)
+    t1, s1, src1, c1 = A.scatter_plan(padded)
+    assert torch.equal(t1 // P, t0 // N) and torch.equal(t1 % P, t0 % N)
+    assert torch.equal(s0, s1) and torch.equal(src0, src1) and torch.equal(c0, c1)

@@ -13,7 +13,7 @@ HEADER = os.path.join(ROOT, "include", "lbm_hip.h")
 def declared_functions():
     text = open(HEADER).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"^\s*(?:int|size_t)\s+(lbm_\w+)\s*\(", text, flags=re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t|size_t)\s+(lbm_\w+)\s*\(", text, flags=re.M)))
 
 
 def test_header_declares_entry_points():
@@ -40,14 +40,38 @@ def test_binding_signatures_cover_header():
     assert not missing, missing
 
 
-def test_step_desc_layout_matches_header():
-    """ctypes mirror of lbm_step_desc: 10 int32, 2 doubles, u_in[3], rho_out,
-    u_ref[3], offset[19], 4 pointers."""
+def _c_offsets(struct, fields):
+    """offsetof/sizeof of a header struct, from a C program compiled with gcc."""
+    import shutil
+    import subprocess
+    import tempfile
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "lbm_hip.h"', 'int main(void) {']
+    for f in fields:
+        lines.append(f'  printf("%zu\\n", offsetof({struct}, {f}));')
+    lines.append(f'  printf("%zu\\n", sizeof({struct}));')
+    lines.append('  return 0; }')
+    with tempfile.TemporaryDirectory() as td:
+        src = os.path.join(td, "layout.c")
+        exe = os.path.join(td, "layout")
+        with open(src, "w") as fh:
+            fh.write("\n".join(lines) + "\n")
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), src, "-o", exe], check=True)
+        out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split()
+    return [int(v) for v in out]
+
+
+@pytest.mark.parametrize("cls,struct", [("StepDesc", "lbm_step_desc"), ("AdjointDesc", "lbm_adjoint_desc")])
+def test_desc_layout_matches_header(cls, struct):
+    """Every field of the ctypes mirrors sits at the C compiler's offset."""
     import lbm_jax_autodiff_amd._lib as L
-    d = L.StepDesc
-    assert d.tau.offset == 40
-    assert d.offset.offset == 40 + 8 * (2 + 3 + 1 + 3)
-    assert ctypes.sizeof(d) == d.link_q.offset + 8
+    import lbm_jax_autodiff_amd.adjoint as A
+    d = getattr(L, cls, None) or getattr(A, cls)
+    names = [f[0] for f in d._fields_]
+    offs = _c_offsets(struct, names)
+    assert offs[:-1] == [getattr(d, n).offset for n in names]
+    assert offs[-1] == ctypes.sizeof(d)
 
 
 def test_no_cpu_fallback_without_device():

@@ -140,3 +140,81 @@ def test_poiseuille_kat(dev):
     published = {0: 0.000200, 1: 0.008247, 5: 0.057873, 9: 0.076879, 10: 0.076879, 15: 0.048352, 19: 0.000200}
     for y, v in published.items():
         assert abs(float(prof[y]) - v) <= 6e-7, (y, float(prof[y]), v)
+
+
+@pytest.mark.parametrize("vec", [1, 4])
+@pytest.mark.parametrize("storage", ["f32", "f16", "f64"])
+def test_population_stride_does_not_change_results(dev, storage, vec):
+    """The padded population stride (default) and the dense layout (pop_stride
+    = N) give bit-identical states and forces."""
+    steps = 5
+    f0 = near_eq_state(NX, NY, NZ, amp=2e-3)
+    q = random_q(NX, NY, NZ, seed=4, p_wall=0.1)
+    g = torch.Generator().manual_seed(2)
+    solid = (torch.rand((NX, NY, NZ), generator=g) < 0.1).to(torch.int32)
+    dt = torch.float64 if storage == "f64" else torch.float32
+    outs = []
+    for stride in (None, NX * NY * NZ, NX * NY * NZ + 4 * 37):
+        lat = Lattice(NX, NY, NZ, tau=0.6, collision="mrt", solid=solid.to(dev), q=q.to(dev), guard="pre",
+                      storage=storage, device=dev, cells_per_thread=vec, pop_stride=stride)
+        outs.append([cpu(t) for t in lat.run(f0.to(dt).to(dev), steps)])
+    assert outs[0][0].shape == (19, NX, NY, NZ)
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            assert torch.equal(a, b)
+
+
+def test_default_population_stride_is_padded(dev):
+    lat = Lattice(256, 128, 128, tau=0.6, device=dev)
+    assert lat.P > lat.N and lat.P % 8192 == 1024
+    buf = lat.new_state()
+    assert buf.stride() == (lat.P, 256 * 128, 256, 1)
+
+
+def _bench_like_lattice(dev, storage="f32"):
+    solid = torch.zeros((NX, NY, NZ), dtype=torch.int32)
+    solid[8:12, 3:6, 2:4] = 1
+    q = O.compute_q_from_solid(solid, NX, NY, NZ, 0.4)
+    return Lattice(NX, NY, NZ, tau=0.6, collision="mrt", solid=solid.to(dev), q=q.to(dev), guard="pre",
+                   storage=storage, device=dev)
+
+
+def test_step_on_user_stream_matches_default_stream(dev):
+    """The link launches fork from and join back into the caller's stream."""
+    f0 = near_eq_state(NX, NY, NZ, amp=2e-3).to(torch.float32)
+    lat = _bench_like_lattice(dev)
+    assert lat.links.n > 0
+    ref = [cpu(t) for t in lat.run(f0.to(dev), 7)]
+    side = torch.cuda.Stream(device=dev)
+    with torch.cuda.stream(side):
+        out = lat.run(f0.to(dev), 7)
+        got = [t.to("cpu") for t in out]
+    torch.cuda.synchronize()
+    for a, b in zip(ref, got):
+        assert torch.equal(a, b)
+
+
+def test_step_graph_replays_match_eager(dev):
+    """One fused step captured in a graph (fork/join included) replays
+    identically; each replay is checked against eager steps."""
+    f0 = near_eq_state(NX, NY, NZ, amp=2e-3).to(torch.float32)
+    lat = _bench_like_lattice(dev)
+    s0 = lat.to_state(f0.to(dev))
+    a, b = lat.new_state(), lat.new_state()
+    lat.collide(s0, a)
+    fp = torch.zeros((max(lat.slots, 1), 3), dtype=torch.float64, device=dev)
+    ea, eb = lat.clone_state(a), lat.new_state()
+    efp = torch.zeros_like(fp)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        lat.step(a, b, fp)
+    for rep in range(4):
+        g.replay()
+        lat.step(ea, eb, efp)  # eager reference of the same step
+        torch.cuda.synchronize()
+        assert torch.equal(cpu(b), cpu(eb)), rep
+        assert torch.equal(cpu(fp), cpu(efp)), rep
+        a.copy_(b)
+        ea.copy_(eb)
+        torch.cuda.synchronize()
