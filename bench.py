@@ -60,18 +60,14 @@ def time_forward(lat, f0, steps, warmup):
     a, b = lat.new_state(), lat.new_state()
     lat.collide(s, a)
     fpart = torch.zeros((max(steps, warmup), max(lat.slots, 1), 3), dtype=torch.float64, device=lat.device)
-    cur, nxt = a, b
-    for t in range(warmup):
-        lat.step(cur, nxt, fpart[t] if lat.slots else None)
-        cur, nxt = nxt, cur
+    cur = lat.pingpong(a, a, b, warmup, fpart[:warmup] if lat.slots else None)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record()
-    for t in range(steps):
-        lat.step(cur, nxt, fpart[t] if lat.slots else None)
-        cur, nxt = nxt, cur
+    # the K timed steps are one host call (lbm_step_chain), as a scan loop would be
+    cur = lat.pingpong(cur, a, b, steps, fpart[:steps] if lat.slots else None)
     e1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -79,6 +75,7 @@ def time_forward(lat, f0, steps, warmup):
     # per-launch kernel durations (events on the launch stream bracket each launch)
     n_k = min(steps, 50)
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(n_k)]
+    nxt = b if cur is a else a
     for t in range(n_k):
         evs[t][0].record()
         lat.step(cur, nxt, fpart[t] if lat.slots else None)

@@ -255,6 +255,16 @@ int lbm_step_force_slots(const lbm_step_desc* d);
 int lbm_step(const lbm_step_desc* d, int mode, const void* src, void* dst,
              double* force_partial, lbm_stream_t stream);
 
+/* n consecutive LBM_MODE_FULL steps in one call: states[k] -> states[k+1]
+ * for k = 0..n-1 (n+1 device buffers, host array; ping-pong by repeating
+ * two pointers).  force_partial (optional) receives step k's partials at
+ * force_partial + k * partial_stride doubles.  Replaces the body of the
+ * reference's jax.lax.scan time loops (examples/10_cd_convergence.py:69-83,
+ * examples/11_optimisation_v2.py:152-160, jax_lbm/state.py:174-230) driven
+ * one launch per host call. */
+int lbm_step_chain(const lbm_step_desc* d, int n, void* const* states, double* force_partial,
+                   int64_t partial_stride, lbm_stream_t stream);
+
 /* force[t][0:3] = fixed-order float64 sum of the `slots` partials of row t,
  * for T rows.  Completes momentum_exchange (forces.py:37-120) for the fused
  * path. */
@@ -278,12 +288,17 @@ typedef struct lbm_adjoint_desc {
   const int32_t* tgt_start;/* n_targets + 1: CSR offsets into src/coef */
   const int32_t* src;      /* >= 0: mu_link index i*n_links + b; < 0: force term of direction -1-src */
   const double* coef;      /* multiplier of mu_link[src] */
+  int32_t lam_storage;     /* cotangent buffers: 0 = the arithmetic type (float,
+                              double for LBM_F64); LBM_F16DEV = IEEE binary16
+                              holding lambda * lam_scale (float arithmetic only) */
+  double lam_scale;        /* power of two for LBM_F16DEV (<= 0 means 1) */
 } lbm_adjoint_desc;
 
-/* One adjoint step (one launch).  mode LBM_MODE_FULL: state = P_t,
+/* One adjoint step.  mode LBM_MODE_FULL: state = P_t,
  * lam_in = dL/dP_{t+1} -> lam_out = owner part of dL/dP_t.  mode
  * LBM_MODE_STREAM: lam_in = dL/dS_{t+1} (the epilogue).  mode
- * LBM_MODE_COLLIDE: state = S_0, lam_in = dL/dP_0 -> lam_out = dL/dS_0.
+ * LBM_MODE_COLLIDE: state = S_0, lam_in = dL/dP_0 -> lam_out = dL/dS_0 (in the
+ * arithmetic type, whatever lam_storage is).
  * gF (3 doubles on the device, may be null) = dL/dF_t of this step's
  * momentum-exchange force.  Requires periodic_yz and nx >= 2.  Must be
  * followed by lbm_adjoint_scatter for modes FULL and STREAM. */
@@ -294,6 +309,16 @@ int lbm_step_adjoint(const lbm_step_desc* d, const lbm_adjoint_desc* a, int mode
 /* Adds the Bouzidi-link and force-term cotangents to lam (same step's gF). */
 int lbm_adjoint_scatter(const lbm_step_desc* d, const lbm_adjoint_desc* a, void* lam,
                         const double* gF, lbm_stream_t stream);
+
+/* Adjoint of n consecutive FULL steps: for k = n-1 down to 0,
+ * lbm_step_adjoint(FULL, states[k]) followed by lbm_adjoint_scatter, with
+ * gF + 3k the force cotangent of step k.  On entry lam_a holds the
+ * cotangent of the state after the last step; the two buffers alternate, so
+ * the result (the cotangent of states[0]) is in lam_a for even n and in
+ * lam_b for odd n.  The transpose of the reference's scan loop under
+ * jax.grad (examples/11_optimisation_v2.py:176). */
+int lbm_adjoint_chain(const lbm_step_desc* d, const lbm_adjoint_desc* a, int n, const void* const* states,
+                      void* lam_a, void* lam_b, const double* gF, lbm_stream_t stream);
 
 /* Convert between storage formats (e.g. float32 <-> half deviations) for a
  * full state; src and dst both use d->pop_stride. */

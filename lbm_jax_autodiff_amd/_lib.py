@@ -80,6 +80,9 @@ _SIGS = {
                                 c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
     "lbm_step_force_slots": (c_int, [ctypes.POINTER(StepDesc)]),
     "lbm_state_stride": (ctypes.c_int64, [ctypes.POINTER(StepDesc)]),
+    "lbm_step_chain": (c_int, [ctypes.POINTER(StepDesc), c_int, c_void_p, c_void_p, ctypes.c_int64, c_void_p]),
+    "lbm_adjoint_chain": (c_int, [ctypes.POINTER(StepDesc), c_void_p, c_int, c_void_p, c_void_p, c_void_p,
+                                  c_void_p, c_void_p]),
     "lbm_step": (c_int, [ctypes.POINTER(StepDesc), c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "lbm_force_reduce": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "lbm_convert_state": (c_int, [ctypes.POINTER(StepDesc), c_int, c_void_p, c_int, c_void_p, c_void_p]),
@@ -130,3 +133,8 @@ def stream_handle(device=None) -> int:
 
 def ptr(t) -> int:
     return 0 if t is None else t.data_ptr()
+
+
+def ptr_array(tensors):
+    """Host array of device pointers (``void* const*`` arguments)."""
+    return (c_void_p * len(tensors))(*[t.data_ptr() for t in tensors])

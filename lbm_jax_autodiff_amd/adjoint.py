@@ -19,6 +19,7 @@ Recomputation is bit-identical to the forward (same kernels, no atomics).
 from __future__ import annotations
 
 import ctypes
+import math
 from typing import Optional, Sequence, Union
 
 import numpy as np
@@ -41,6 +42,7 @@ class AdjointDesc(ctypes.Structure):
         ("dq_acc", ctypes.c_void_p), ("mu_link", ctypes.c_void_p), ("phi_acc", ctypes.c_void_p),
         ("u_acc", ctypes.c_void_p), ("n_targets", ctypes.c_int32), ("tgt_index", ctypes.c_void_p),
         ("tgt_start", ctypes.c_void_p), ("src", ctypes.c_void_p), ("coef", ctypes.c_void_p),
+        ("lam_storage", ctypes.c_int32), ("lam_scale", ctypes.c_double),
     ]
 
 
@@ -143,40 +145,41 @@ class _Checkpoints:
     def _recompute(self, p_start, t0, t1, keep_every):
         """{t: P_t} for t = t0 and every t0 < t < t1 with t % keep_every == 0, from P_t0."""
         lat = self.lat
+        if keep_every == 1:
+            bufs = [p_start] + [lat.new_state() for _ in range(t1 - 1 - t0)]
+            lat.chain(bufs)
+            return {t0 + k: b for k, b in enumerate(bufs)}
         out = {t0: p_start}
-        cur = p_start
-        sa = sb = None
-        if keep_every > 1:
-            sa, sb = lat.new_state(), lat.new_state()
-        for t in range(t0, t1 - 1):
-            if keep_every == 1:
-                dst = lat.new_state()
-                lat.step(cur, dst)
-                out[t + 1] = dst
-                cur = dst
-            else:
-                dst = sa if cur is not sa else sb
-                lat.step(cur, dst)
-                if (t + 1) % keep_every == 0:
-                    out[t + 1] = lat.clone_state(dst)
-                cur = dst
+        sa, sb = lat.new_state(), lat.new_state()
+        cur, t = p_start, t0
+        while t < t1 - 1:
+            end = min(t1 - 1, (t // keep_every + 1) * keep_every)
+            cur = lat.pingpong(cur, sa, sb, end - t)
+            if end % keep_every == 0:
+                out[end] = lat.clone_state(cur)
+            t = end
         return out
 
-    def descending(self):
-        """Generator of (t, P_t) for t from N-1 down to 0."""
-        yield from self._desc(self.top, 0, self.N, 0)
+    def segments(self):
+        """(t0, t1, {t: P_t for t0 <= t < t1}) for consecutive segments, last first."""
+        yield from self._seg(self.top, 0, self.N, 0)
 
-    def _desc(self, have, t0, t1, level):
+    def _seg(self, have, t0, t1, level):
         if self.levels[level] == 1:
-            for t in range(t1 - 1, t0 - 1, -1):
-                yield t, have[t]
+            yield t0, t1, have
             return
         starts = sorted(t for t in have if t0 <= t < t1)
         ends = starts[1:] + [t1]
         for s, e in reversed(list(zip(starts, ends))):
             sub = self._recompute(have[s], s, e, self.levels[level + 1])
-            yield from self._desc(sub, s, e, level + 1)
+            yield from self._seg(sub, s, e, level + 1)
             del sub
+
+    def descending(self):
+        """Generator of (t, P_t) for t from N-1 down to 0."""
+        for t0, t1, have in self.segments():
+            for t in range(t1 - 1, t0 - 1, -1):
+                yield t, have[t]
 
 
 class _RolloutFn(torch.autograd.Function):
@@ -192,17 +195,26 @@ class _RolloutFn(torch.autograd.Function):
                       offset=cfg["offset"], device=f0.device)
         ck = _Checkpoints(lat, cfg["checkpoint"], n_steps)
         s0 = lat.to_state(f0.detach())
-        a, b = lat.new_state(), lat.new_state()
+        a = lat.new_state()
         fpart = torch.zeros((n_steps, max(lat.slots, 1), 3), dtype=torch.float64, device=lat.device)
         lat.collide(s0, a)
-        cur, nxt = a, b
-        for t in range(n_steps - 1):
-            ck.record(t, cur)
-            if ck.store_all:
-                nxt = lat.new_state()  # cur is kept: write P_{t+1} to a fresh buffer
-            lat.step(cur, nxt, fpart[t] if lat.slots else None)
-            cur, nxt = nxt, cur
-        ck.record(n_steps - 1, cur)
+        fp = fpart if lat.slots else None
+        if ck.store_all:
+            # every P_t in a buffer of its own
+            bufs = [a] + [lat.new_state() for _ in range(n_steps - 1)]
+            lat.chain(bufs, fp)
+            for t, buf in enumerate(bufs):
+                ck.record(t, buf)
+            cur = bufs[-1]
+        else:
+            b = lat.new_state()
+            cur, t = a, 0
+            while t < n_steps - 1:
+                ck.record(t, cur)
+                end = min(n_steps - 1, (t // ck.stride + 1) * ck.stride)
+                cur = lat.pingpong(cur, a, b, end - t, None if fp is None else fp[t:end])
+                t = end
+            ck.record(n_steps - 1, cur)
         s_final = lat.new_state()
         lat.stream(cur, s_final, fpart[n_steps - 1] if lat.slots else None)
         forces = lat.reduce_forces(fpart, n_steps) if lat.slots else torch.zeros(
@@ -265,34 +277,55 @@ class _RolloutFn(torch.autograd.Function):
         dptr = ctypes.byref(lat.desc)
         aptr = ctypes.byref(ad)
 
-        lam_a = lat.new_state(real)
-        lam_b = lat.new_state(real)
-        g_in = lat.new_state(real)
+        lam_dtype, scale = real, 1.0
+        if ctx.cfg["adjoint_storage"] == "f16":
+            # lambda stored as IEEE half times a power of two that puts the
+            # largest seed at 2^12 (16x headroom below the half maximum)
+            seed = 3.0 * float(gF.abs().max()) if gF.numel() else 0.0
+            for g in (gS, gP):
+                if g is not None:
+                    seed = max(seed, float(g.detach().abs().max()))
+            scale = 2.0 ** (12 - math.ceil(math.log2(seed))) if seed > 0.0 else 1.0
+            lam_dtype = torch.float16
+            ad.lam_storage = L.LBM_F16DEV
+            ad.lam_scale = scale
+        lam_a = lat.new_state(lam_dtype)
+        lam_b = lat.new_state(lam_dtype)
+        g_in = lat.new_state(lam_dtype)
         if gS is not None:
-            g_in.copy_(gS.detach().permute(0, 3, 2, 1))
+            g_in.copy_(gS.detach().permute(0, 3, 2, 1).to(real) * scale)
         else:
             g_in.zero_()
-        it = ck.descending()
-        t, p_t = next(it)
-        assert t == n_steps - 1
-        L.call("lbm_step_adjoint", dptr, aptr, L.LBM_MODE_STREAM, L.ptr(p_t), L.ptr(g_in), L.ptr(lam_a),
-               L.ptr(gF[t]), stream)
-        L.call("lbm_adjoint_scatter", dptr, aptr, L.ptr(lam_a), L.ptr(gF[t]), stream)
-        if gP is not None:
-            lam_a += gP.detach().permute(0, 3, 2, 1).to(real)
         cur, nxt = lam_a, lam_b
-        for t, p_t in it:
-            L.call("lbm_step_adjoint", dptr, aptr, L.LBM_MODE_FULL, L.ptr(p_t), L.ptr(cur), L.ptr(nxt),
-                   L.ptr(gF[t]), stream)
-            L.call("lbm_adjoint_scatter", dptr, aptr, L.ptr(nxt), L.ptr(gF[t]), stream)
-            cur, nxt = nxt, cur
-        # cur = dL/dP_0 -> prologue transpose
+        first = True
+        for t0, t1, have in ck.segments():
+            if first:
+                # epilogue transpose at t = N-1: lam = dL/dS_N -> dL/dP_{N-1}
+                t = n_steps - 1
+                assert t1 == n_steps
+                L.call("lbm_step_adjoint", dptr, aptr, L.LBM_MODE_STREAM, L.ptr(have[t]), L.ptr(g_in), L.ptr(cur),
+                       L.ptr(gF[t]), stream)
+                L.call("lbm_adjoint_scatter", dptr, aptr, L.ptr(cur), L.ptr(gF[t]), stream)
+                if gP is not None:
+                    cur.copy_(cur.to(real) + gP.detach().permute(0, 3, 2, 1).to(real) * scale)
+                t1 -= 1
+                first = False
+            n = t1 - t0
+            if n > 0:
+                L.call("lbm_adjoint_chain", dptr, aptr, n, L.ptr_array([have[t] for t in range(t0, t1)]),
+                       L.ptr(cur), L.ptr(nxt), L.ptr(gF[t0]), stream)
+                if n % 2 == 1:
+                    cur, nxt = nxt, cur
+        if lam_dtype == torch.float16 and not bool(torch.isfinite(cur).all()):
+            raise FloatingPointError("adjoint cotangents left the float16 range; use adjoint_storage='f32'")
+        # cur = dL/dP_0 -> prologue transpose (output in the arithmetic type)
         g_f0 = None
         if want_f0 or want_phi:
-            L.call("lbm_step_adjoint", dptr, aptr, L.LBM_MODE_COLLIDE, L.ptr(ctx.s0), L.ptr(cur), L.ptr(nxt),
+            g_s0 = nxt if lam_dtype == real else lat.new_state(real)
+            L.call("lbm_step_adjoint", dptr, aptr, L.LBM_MODE_COLLIDE, L.ptr(ctx.s0), L.ptr(cur), L.ptr(g_s0),
                    None, stream)
             if want_f0:
-                g_f0 = F.pop_view(nxt).to(ctx.f0_dtype).clone()
+                g_f0 = F.pop_view(g_s0).to(ctx.f0_dtype).clone()
         g_q = None
         if want_q:
             dense = torch.zeros((19, N), dtype=torch.float64, device=dev)
@@ -309,21 +342,29 @@ def rollout(f0: torch.Tensor, n_steps: int, *, tau, q: Optional[torch.Tensor] = 
             inlet_velocity: Union[Sequence[float], torch.Tensor] = (0.05, 0.0, 0.0), rho_out: float = 1.0,
             collision: str = "bgk", smagorinsky_cs: Optional[float] = None, guard: str = "none",
             guard_velocity=None, storage: str = "f32", offset="weights",
-            checkpoint: Union[str, int, Sequence[int]] = "all"):
+            checkpoint: Union[str, int, Sequence[int]] = "all", adjoint_storage: Optional[str] = None):
     """Differentiable fused rollout: (S_N, P_{N-1}, forces[N, 3] float64).
 
     Step order as in the reference's drivers: [pre guard] -> [Smagorinsky]
     -> BGK|MRT -> soft bounce (porosity) or hard bounce (solid >= 1) -> stream
     with Bouzidi (q) -> Zou-He inlet/outlet -> [post guard].
+    ``adjoint_storage`` "f32" keeps the cotangents in the arithmetic type,
+    "f16" stores them as scaled IEEE half (half storages only; default for
+    them), halving the adjoint's cotangent traffic.
     """
     nx, ny, nz = F.dims_of_pop(f0)
     if isinstance(inlet_velocity, torch.Tensor):
         u = inlet_velocity.reshape(3)
     else:
         u = torch.tensor([float(v) for v in inlet_velocity], dtype=torch.float64, device=f0.device)
+    if adjoint_storage is None:
+        adjoint_storage = "f16" if storage in ("f16", "bf16") else "f32"
+    if adjoint_storage not in ("f32", "f16") or (adjoint_storage == "f16" and storage == "f64"):
+        raise ValueError("adjoint_storage must be 'f32', or 'f16' with float arithmetic")
     cfg = dict(n_steps=int(n_steps), nx=nx, ny=ny, nz=nz, tau=F.scalar_float(tau), rho_out=float(rho_out),
                collision=collision, smagorinsky_cs=smagorinsky_cs, solid=solid, guard=guard,
-               guard_velocity=guard_velocity, storage=storage, offset=offset, checkpoint=checkpoint)
+               guard_velocity=guard_velocity, storage=storage, offset=offset, checkpoint=checkpoint,
+               adjoint_storage=adjoint_storage)
     if int(n_steps) < 1:
         raise ValueError("n_steps must be >= 1")
     return _RolloutFn.apply(f0, q, porosity, u, cfg)

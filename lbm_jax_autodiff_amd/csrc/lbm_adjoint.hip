@@ -43,6 +43,24 @@ struct AdjArgs {
   R* mu_link;
   double* phi_acc;
   double* u_acc;
+  // cotangent storage scale (half storage) and its inverse
+  R lam_scale, lam_inv;
+};
+
+// Storage of the cotangent (lambda) buffers: the arithmetic type itself, or
+// IEEE binary16 holding lambda * lam_scale (a power of two chosen from the
+// seeds' magnitude), rounded to nearest even.  The adjoint is linear in
+// lambda, so the scale only moves the values into the half range.
+template <typename R>
+struct LamPlain {
+  using T = R;
+  __device__ static R ld(const T* p, size_t k, R) { return p[k]; }
+  __device__ static void st(T* p, size_t k, R v, R) { p[k] = v; }
+};
+struct LamHalf {
+  using T = unsigned short;
+  __device__ static float ld(const T* p, size_t k, float inv_scale) { return f16_bits_to_float(p[k]) * inv_scale; }
+  __device__ static void st(T* p, size_t k, float v, float scale) { p[k] = float_to_f16_bits(v * scale); }
 };
 
 template <typename R>
@@ -176,10 +194,10 @@ __device__ __forceinline__ bool collide_transpose(const AdjArgs<R>& a, R* s, R* 
 
 // One cell: forward recompute + cotangent pull-back + owner writes.
 // MODE FULL: lam_in = dL/dP_{t+1}; MODE STREAM: lam_in = dL/dS_{t+1}.
-template <class S, int COLL, int MODE, bool LINK>
+template <class S, class LM, int COLL, int MODE, bool LINK>
 __device__ __forceinline__ void adjoint_cell(const AdjArgs<typename S::R>& a, const typename S::T* __restrict__ P,
-                                             const typename S::R* __restrict__ lam_in,
-                                             typename S::R* __restrict__ lam_out, const double* __restrict__ gF,
+                                             const typename LM::T* __restrict__ lam_in,
+                                             typename LM::T* __restrict__ lam_out, const double* __restrict__ gF,
                                              uint32_t c, int b, uint32_t fl, const typename S::R* qv) {
   using R = typename S::R;
   const size_t N = a.P;
@@ -199,7 +217,7 @@ __device__ __forceinline__ void adjoint_cell(const AdjArgs<typename S::R>& a, co
   R g[Q];
   sfor<Q>([&](auto I) {
     constexpr int i = I;
-    g[i] = lam_in[i * N + c];
+    g[i] = LM::ld(lam_in, i * N + c, a.lam_inv);
   });
   if constexpr (MODE == LBM_MODE_FULL) collide_transpose<R, COLL>(a, s, g, c, fl, skip);
   // momentum-exchange force uses S_{t+1}(x)_opp(j) for links j (forces.py:95-118)
@@ -240,7 +258,7 @@ __device__ __forceinline__ void adjoint_cell(const AdjArgs<typename S::R>& a, co
   }
   // owner writes and Bouzidi bookkeeping
   if (skip) return;
-  lam_out[c] = g[0];
+  LM::st(lam_out, c, g[0], a.lam_scale);
   sfor<Q>([&](auto I) {
     constexpr int i = I;
     if constexpr (i != 0) {
@@ -252,7 +270,7 @@ __device__ __forceinline__ void adjoint_cell(const AdjArgs<typename S::R>& a, co
         link = qo >= R(0);
       }
       const bool normal = !link && !x_fallback<i>(x, a.nx);
-      lam_out[i * N + owner_target<i>(a, x, y, z)] = normal ? g[i] : R(0);
+      LM::st(lam_out, i * N + owner_target<i>(a, x, y, z), normal ? g[i] : R(0), a.lam_scale);
       if constexpr (LINK) {
         a.mu_link[size_t(i) * a.n_links + b] = link ? g[i] : R(0);
         if (link && a.dq_acc) {
@@ -272,11 +290,11 @@ __device__ __forceinline__ void adjoint_cell(const AdjArgs<typename S::R>& a, co
 
 // Link cells (Bouzidi transpose, mu_link, dq): a small launch ahead of the
 // dense sweep, as in the forward step.
-template <class S, int COLL, int MODE>
+template <class S, class LM, int COLL, int MODE>
 __global__ __launch_bounds__(kAdjLinkBlock) void k_adjoint_links(AdjArgs<typename S::R> a,
                                                                  const typename S::T* __restrict__ P,
-                                                                 const typename S::R* __restrict__ lam_in,
-                                                                 typename S::R* __restrict__ lam_out,
+                                                                 const typename LM::T* __restrict__ lam_in,
+                                                                 typename LM::T* __restrict__ lam_out,
                                                                  const double* __restrict__ gF) {
   using R = typename S::R;
   const int b = int(blockIdx.x) * kAdjLinkBlock + int(threadIdx.x);
@@ -289,7 +307,7 @@ __global__ __launch_bounds__(kAdjLinkBlock) void k_adjoint_links(AdjArgs<typenam
     constexpr int i = I;
     if constexpr (i != 0) qv[i] = R(a.link_q[size_t(i - 1) * a.n_links + b]);
   });
-  adjoint_cell<S, COLL, MODE, true>(a, P, lam_in, lam_out, gF, c, b, a.flags[c], qv);
+  adjoint_cell<S, LM, COLL, MODE, true>(a, P, lam_in, lam_out, gF, c, b, a.flags[c], qv);
 }
 
 // Occupancy target of the dense adjoint sweep: the FP32 MRT transpose fits
@@ -301,23 +319,22 @@ constexpr int adj_waves_per_eu() {
 }
 
 // Dense sweep; link cells are recomputed but written by k_adjoint_links.
-template <class S, int COLL, int MODE>
-__global__ __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(adj_waves_per_eu<S, COLL>()))) void k_adjoint(AdjArgs<typename S::R> a, const typename S::T* __restrict__ P,
-                                                       const typename S::R* __restrict__ lam_in,
-                                                       typename S::R* __restrict__ lam_out,
-                                                       const double* __restrict__ gF) {
+template <class S, class LM, int COLL, int MODE>
+__global__ __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(adj_waves_per_eu<S, COLL>()))) void k_adjoint(
+    AdjArgs<typename S::R> a, const typename S::T* __restrict__ P, const typename LM::T* __restrict__ lam_in,
+    typename LM::T* __restrict__ lam_out, const double* __restrict__ gF) {
   const uint32_t tile = xcd_swizzle(blockIdx.x, uint32_t(a.ndb));
   const uint32_t c = tile * kAdjBlock + threadIdx.x;
   if (c >= a.N) return;
-  adjoint_cell<S, COLL, MODE, false>(a, P, lam_in, lam_out, gF, c, -1, a.flags ? uint32_t(a.flags[c]) : 0u,
+  adjoint_cell<S, LM, COLL, MODE, false>(a, P, lam_in, lam_out, gF, c, -1, a.flags ? uint32_t(a.flags[c]) : 0u,
                                      nullptr);
 }
 
 // Prologue transpose: dL/dS_0 = ([pre guard] collide bounce)^T lambda_0, cell-local.
-template <class S, int COLL>
+template <class S, class LM, int COLL>
 __global__ __launch_bounds__(kAdjBlock) void k_adjoint_collide(AdjArgs<typename S::R> a,
                                                                const typename S::T* __restrict__ S0,
-                                                               const typename S::R* __restrict__ lam_in,
+                                                               const typename LM::T* __restrict__ lam_in,
                                                                typename S::R* __restrict__ g_out) {
   using R = typename S::R;
   const uint32_t c = blockIdx.x * kAdjBlock + threadIdx.x;
@@ -327,7 +344,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjoint_collide(AdjArgs<typename 
   sfor<Q>([&](auto I) {
     constexpr int i = I;
     s[i] = S::ld_c(S0, i * N + c, a.off[i]);
-    g[i] = lam_in[i * N + c];
+    g[i] = LM::ld(lam_in, i * N + c, a.lam_inv);
   });
   collide_transpose<R, COLL>(a, s, g, c, a.flags ? uint32_t(a.flags[c]) : 0u, false);
   sfor<Q>([&](auto I) {
@@ -338,10 +355,11 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjoint_collide(AdjArgs<typename 
 
 // Deterministic additions: lam[tgt[k]] += sum_c coef[c] * mu_link[src[c]]
 // (+ gF . e_j for force terms, encoded src = -1 - j).
-template <typename R>
+template <typename R, class LM>
 __global__ void k_adjoint_scatter(int n_targets, const int32_t* __restrict__ tgt, const int32_t* __restrict__ start,
                                   const int32_t* __restrict__ src, const double* __restrict__ coef,
-                                  const R* __restrict__ mu_link, const double* __restrict__ gF, R* __restrict__ lam) {
+                                  const R* __restrict__ mu_link, const double* __restrict__ gF,
+                                  typename LM::T* __restrict__ lam, R lam_scale, R lam_inv) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n_targets) return;
   R acc = R(0);
@@ -359,7 +377,8 @@ __global__ void k_adjoint_scatter(int n_targets, const int32_t* __restrict__ tgt
       acc += v;
     }
   }
-  lam[tgt[k]] += acc;
+  const size_t t = size_t(tgt[k]);
+  LM::st(lam, t, LM::ld(lam, t, lam_inv) + acc, lam_scale);
 }
 
 template <typename R>
@@ -397,6 +416,9 @@ int make_adj_args(const lbm_step_desc* d, const lbm_adjoint_desc* ad, int mode, 
   a.mu_link = static_cast<R*>(ad->mu_link);
   a.phi_acc = ad->phi_acc;
   a.u_acc = ad->u_acc;
+  const double scale = (ad->lam_storage == LBM_F16DEV && ad->lam_scale > 0.0) ? ad->lam_scale : 1.0;
+  a.lam_scale = R(scale);
+  a.lam_inv = R(1.0 / scale);
   if (a.bounce == LBM_BOUNCE_HARD && !a.flags) return LBM_E_ARG;
   if (a.bounce == LBM_BOUNCE_SOFT && !a.phi) return LBM_E_ARG;
   if (a.n_links < 0 || (a.n_links > 0 && (!a.link_cell || !a.link_q || !a.flags || !a.mu_link))) return LBM_E_ARG;
@@ -405,7 +427,7 @@ int make_adj_args(const lbm_step_desc* d, const lbm_adjoint_desc* ad, int mode, 
   return LBM_OK;
 }
 
-template <class S>
+template <class S, class LM>
 int launch_adjoint(const lbm_step_desc* d, const lbm_adjoint_desc* ad, int mode, const void* state,
                    const void* lam_in, void* lam_out, const double* gF, hipStream_t st) {
   using R = typename S::R;
@@ -414,23 +436,25 @@ int launch_adjoint(const lbm_step_desc* d, const lbm_adjoint_desc* ad, int mode,
   if (rc) return rc;
   if (!state || !lam_in || !lam_out || lam_in == lam_out) return LBM_E_ARG;
   const auto* P = static_cast<const typename S::T*>(state);
-  const auto* li = static_cast<const R*>(lam_in);
-  auto* lo = static_cast<R*>(lam_out);
+  const auto* li = static_cast<const typename LM::T*>(lam_in);
   const bool mrt = d->collision == LBM_COLL_MRT;
   if (mode == LBM_MODE_COLLIDE) {
+    // the prologue's output dL/dS_0 is in the arithmetic type
+    auto* go = static_cast<R*>(lam_out);
     const unsigned grid = unsigned(a.ndb);
-    if (mrt) hipLaunchKernelGGL((k_adjoint_collide<S, LBM_COLL_MRT>), grid, kAdjBlock, 0, st, a, P, li, lo);
-    else hipLaunchKernelGGL((k_adjoint_collide<S, LBM_COLL_BGK>), grid, kAdjBlock, 0, st, a, P, li, lo);
+    if (mrt) hipLaunchKernelGGL((k_adjoint_collide<S, LM, LBM_COLL_MRT>), grid, kAdjBlock, 0, st, a, P, li, go);
+    else hipLaunchKernelGGL((k_adjoint_collide<S, LM, LBM_COLL_BGK>), grid, kAdjBlock, 0, st, a, P, li, go);
   } else {
+    auto* lo = static_cast<typename LM::T*>(lam_out);
     const unsigned grid = unsigned(a.ndb);
 #define LBM_ADJ(COLL, MODE)                                                                                     \
   do {                                                                                                          \
     auto dense = [&](hipStream_t q) {                                                                           \
-      hipLaunchKernelGGL((k_adjoint<S, COLL, MODE>), grid, kAdjBlock, 0, q, a, P, li, lo, gF);                  \
+      hipLaunchKernelGGL((k_adjoint<S, LM, COLL, MODE>), grid, kAdjBlock, 0, q, a, P, li, lo, gF);                  \
     };                                                                                                          \
     if (a.nlb > 0) {                                                                                            \
       auto links = [&](hipStream_t q) {                                                                         \
-        hipLaunchKernelGGL((k_adjoint_links<S, COLL, MODE>), unsigned(a.nlb), kAdjLinkBlock, 0, q, a, P, li, lo, gF); \
+        hipLaunchKernelGGL((k_adjoint_links<S, LM, COLL, MODE>), unsigned(a.nlb), kAdjLinkBlock, 0, q, a, P, li, lo, gF); \
       };                                                                                                        \
       links(st);                                                                                                \
       dense(st);                                                                                                \
@@ -462,11 +486,21 @@ int lbm_step_adjoint(const lbm_step_desc* d, const lbm_adjoint_desc* ad, int mod
                      const void* lam_in, void* lam_out, const double* gF, lbm_stream_t stream) {
   if (!d) return LBM_E_ARG;
   hipStream_t s = (hipStream_t)stream;
+  if (!ad) return LBM_E_ARG;
+  if (ad->lam_storage == LBM_F16DEV) {
+    switch (d->storage) {
+      case LBM_F32: return launch_adjoint<StoreF32, LamHalf>(d, ad, mode, state, lam_in, lam_out, gF, s);
+      case LBM_F16DEV: return launch_adjoint<StoreF16, LamHalf>(d, ad, mode, state, lam_in, lam_out, gF, s);
+      case LBM_BF16DEV: return launch_adjoint<StoreBF16, LamHalf>(d, ad, mode, state, lam_in, lam_out, gF, s);
+    }
+    return LBM_E_DTYPE;
+  }
+  if (ad->lam_storage != 0) return LBM_E_DTYPE;
   switch (d->storage) {
-    case LBM_F32: return launch_adjoint<StoreF32>(d, ad, mode, state, lam_in, lam_out, gF, s);
-    case LBM_F64: return launch_adjoint<StoreF64>(d, ad, mode, state, lam_in, lam_out, gF, s);
-    case LBM_F16DEV: return launch_adjoint<StoreF16>(d, ad, mode, state, lam_in, lam_out, gF, s);
-    case LBM_BF16DEV: return launch_adjoint<StoreBF16>(d, ad, mode, state, lam_in, lam_out, gF, s);
+    case LBM_F32: return launch_adjoint<StoreF32, LamPlain<float>>(d, ad, mode, state, lam_in, lam_out, gF, s);
+    case LBM_F64: return launch_adjoint<StoreF64, LamPlain<double>>(d, ad, mode, state, lam_in, lam_out, gF, s);
+    case LBM_F16DEV: return launch_adjoint<StoreF16, LamPlain<float>>(d, ad, mode, state, lam_in, lam_out, gF, s);
+    case LBM_BF16DEV: return launch_adjoint<StoreBF16, LamPlain<float>>(d, ad, mode, state, lam_in, lam_out, gF, s);
   }
   return LBM_E_DTYPE;
 }
@@ -478,14 +512,43 @@ int lbm_adjoint_scatter(const lbm_step_desc* d, const lbm_adjoint_desc* ad, void
   if (!ad->tgt_index || !ad->tgt_start || !ad->src || !ad->coef || !ad->mu_link) return LBM_E_ARG;
   const unsigned grid = unsigned((ad->n_targets + 255) / 256);
   hipStream_t s = (hipStream_t)stream;
-  if (d->storage == LBM_F64) {
-    hipLaunchKernelGGL(k_adjoint_scatter<double>, grid, 256, 0, s, ad->n_targets, ad->tgt_index, ad->tgt_start,
-                       ad->src, ad->coef, static_cast<const double*>(ad->mu_link), gF, static_cast<double*>(lam));
+  if (ad->lam_storage == LBM_F16DEV) {
+    if (d->storage == LBM_F64) return LBM_E_DTYPE;
+    const float scale = ad->lam_scale > 0.0 ? float(ad->lam_scale) : 1.0f;
+    hipLaunchKernelGGL((k_adjoint_scatter<float, LamHalf>), grid, 256, 0, s, ad->n_targets, ad->tgt_index,
+                       ad->tgt_start, ad->src, ad->coef, static_cast<const float*>(ad->mu_link), gF,
+                       static_cast<unsigned short*>(lam), scale, 1.0f / scale);
+  } else if (ad->lam_storage != 0) {
+    return LBM_E_DTYPE;
+  } else if (d->storage == LBM_F64) {
+    hipLaunchKernelGGL((k_adjoint_scatter<double, LamPlain<double>>), grid, 256, 0, s, ad->n_targets, ad->tgt_index,
+                       ad->tgt_start, ad->src, ad->coef, static_cast<const double*>(ad->mu_link), gF,
+                       static_cast<double*>(lam), 1.0, 1.0);
   } else {
-    hipLaunchKernelGGL(k_adjoint_scatter<float>, grid, 256, 0, s, ad->n_targets, ad->tgt_index, ad->tgt_start,
-                       ad->src, ad->coef, static_cast<const float*>(ad->mu_link), gF, static_cast<float*>(lam));
+    hipLaunchKernelGGL((k_adjoint_scatter<float, LamPlain<float>>), grid, 256, 0, s, ad->n_targets, ad->tgt_index,
+                       ad->tgt_start, ad->src, ad->coef, static_cast<const float*>(ad->mu_link), gF,
+                       static_cast<float*>(lam), 1.0f, 1.0f);
   }
   return hipGetLastError() == hipSuccess ? LBM_OK : LBM_E_LAUNCH;
+}
+
+
+int lbm_adjoint_chain(const lbm_step_desc* d, const lbm_adjoint_desc* ad, int n, const void* const* states,
+                      void* lam_a, void* lam_b, const double* gF, lbm_stream_t stream) {
+  if (!d || !ad || n < 0 || (n > 0 && (!states || !lam_a || !lam_b)) || lam_a == lam_b) return LBM_E_ARG;
+  void* cur = lam_a;
+  void* nxt = lam_b;
+  for (int k = n - 1; k >= 0; --k) {
+    const double* g = gF ? gF + 3 * size_t(k) : nullptr;
+    int rc = lbm_step_adjoint(d, ad, LBM_MODE_FULL, states[k], cur, nxt, g, stream);
+    if (rc) return rc;
+    rc = lbm_adjoint_scatter(d, ad, nxt, g, stream);
+    if (rc) return rc;
+    void* t = cur;
+    cur = nxt;
+    nxt = t;
+  }
+  return LBM_OK;
 }
 
 }  // extern "C"

@@ -576,6 +576,17 @@ int lbm_step(const lbm_step_desc* d, int mode, const void* src, void* dst, doubl
   return LBM_E_DTYPE;
 }
 
+int lbm_step_chain(const lbm_step_desc* d, int n, void* const* states, double* force_partial,
+                   int64_t partial_stride, lbm_stream_t stream) {
+  if (!d || n < 0 || (n > 0 && !states) || partial_stride < 0) return LBM_E_ARG;
+  for (int k = 0; k < n; ++k) {
+    double* fp = force_partial ? force_partial + size_t(k) * size_t(partial_stride) : nullptr;
+    const int rc = lbm_step(d, LBM_MODE_FULL, states[k], states[k + 1], fp, stream);
+    if (rc) return rc;
+  }
+  return LBM_OK;
+}
+
 int lbm_force_reduce(const double* partials, int T, int slots, double* force, lbm_stream_t stream) {
   if (T <= 0 || !force || slots < 0 || (slots > 0 && !partials)) return LBM_E_ARG;
   hipLaunchKernelGGL(k_force_reduce, (T + 255) / 256, 256, 0, (hipStream_t)stream, partials, T, slots, force);

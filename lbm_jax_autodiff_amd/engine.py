@@ -244,6 +244,31 @@ class Lattice:
         """S_{t+1} from P_t (rollout epilogue)."""
         self.launch(L.LBM_MODE_STREAM, p_in, s_out, fpart)
 
+    def chain(self, states, fpart: Optional[torch.Tensor] = None):
+        """len(states) - 1 full steps in one host call: states[k] -> states[k+1].
+        fpart, if given, is a contiguous (n, slots, 3) float64 block receiving
+        each step's force partials."""
+        n = len(states) - 1
+        if n <= 0:
+            return
+        fp, stride = 0, 0
+        if fpart is not None and self.slots > 0:
+            assert fpart.is_contiguous() and fpart.shape[0] >= n
+            fp, stride = fpart.data_ptr(), 3 * self.slots
+        L.call("lbm_step_chain", ctypes.byref(self.desc), n, L.ptr_array(states), fp, stride,
+               L.stream_handle(self.device))
+
+    def pingpong(self, cur, spare_a, spare_b, n, fpart=None):
+        """n full steps from cur through two spare buffers (never writing cur
+        unless it is one of them); returns the buffer holding the result."""
+        if n <= 0:
+            return cur
+        first = spare_a if cur is not spare_a else spare_b
+        second = spare_b if first is spare_a else spare_a
+        seq = [cur] + [first if k % 2 == 0 else second for k in range(n)]
+        self.chain(seq, fpart)
+        return seq[-1]
+
     def reduce_forces(self, fpart: torch.Tensor, T: int) -> torch.Tensor:
         force = torch.empty((T, 3), dtype=torch.float64, device=self.device)
         if T > 0:
@@ -270,12 +295,16 @@ class Lattice:
         if forces and self.slots > 0:
             fpart = torch.zeros((n_steps, self.slots, 3), dtype=torch.float64, device=self.device)
         self.collide(s0, a)
-        cur, nxt = a, b
-        for t in range(n_steps - 1):
+        cur = a
+        t = 0
+        while t < n_steps - 1:
             if keep is not None and keep_every and t % keep_every == 0:
                 keep.append((t, self.clone_state(cur)))
-            self.step(cur, nxt, None if fpart is None else fpart[t])
-            cur, nxt = nxt, cur
+            end = n_steps - 1
+            if keep is not None and keep_every:
+                end = min(end, (t // keep_every + 1) * keep_every)
+            cur = self.pingpong(cur, a, b, end - t, None if fpart is None else fpart[t:end])
+            t = end
         if keep is not None and keep_every and (n_steps - 1) % keep_every == 0:
             keep.append((n_steps - 1, self.clone_state(cur)))
         s_final = s0  # reuse the prologue input buffer for S_N

@@ -146,7 +146,7 @@ def sphere_cd(nx, ny, nz, n_steps, center_world, radius=0.5, *, init_u=0.0, stor
 def sphere_loss_and_grad(nx, ny, nz, n_steps, center_world, radius=0.5, *, avg_last=600, q_mode="bouzidi",
                          porosity_mode="hard", sharpness=20.0, collision="mrt", re=100.0, u_in=0.05,
                          storage="f32", checkpoint="all", smagorinsky_cs=None, guard="none", init_u=0.05,
-                         offset="inlet", geometry_dtype=None, r_ref=0.5):
+                         offset="inlet", geometry_dtype=None, r_ref=0.5, adjoint_storage=None):
     """loss = mean Cd over the last `avg_last` steps and dloss/dr (world units)
     through q(r) (Bouzidi) and porosity(r) (soft), examples/11:56-167.  The
     initial state is f_eq(1, (init_u, 0, 0)) (examples/11:147-149 use u_inlet).  The
@@ -160,7 +160,13 @@ def sphere_loss_and_grad(nx, ny, nz, n_steps, center_world, radius=0.5, *, avg_l
                                                 coord_dtype=gdt, out_dtype=dt)
     if q_mode == "halfway":
         q = compute_q_from_solid(solid, nx, ny, nz, 0.5, dtype=dt)
-    porosity = por_soft if porosity_mode == "soft" else solid.to(dt) + 0.0 * por_soft
+    # "hard" in examples/11:62-65 is the soft blend fed porosity = solid (0/1,
+    # zero gradient).  With p in {0, 1} the blend equals the hard-mask
+    # reversal bit for bit and its dL/dp is multiplied by zero, so the hard
+    # mask gives the same loss and dCd/dr without the per-cell blend and the
+    # float64 dL/dp accumulation.
+    porosity = por_soft if porosity_mode == "soft" else None
+    hard_solid = solid if porosity_mode == "hard" else None
     D = 2.0 * float(r_ref) * nx / 8.0
     tau = tau_for(u_in, D, re)
     u = torch.zeros((3, nx, ny, nz), dtype=dt, device=dev)
@@ -169,9 +175,11 @@ def sphere_loss_and_grad(nx, ny, nz, n_steps, center_world, radius=0.5, *, avg_l
     area = torch.clamp(projected_area(solid, 0).to(torch.float64), min=1.0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    _, _, forces = rollout(f0, n_steps, tau=tau, q=q, porosity=porosity, inlet_velocity=(u_in, 0.0, 0.0),
+    _, _, forces = rollout(f0, n_steps, tau=tau, q=q, porosity=porosity, solid=hard_solid,
+                           inlet_velocity=(u_in, 0.0, 0.0),
                            collision=collision, smagorinsky_cs=smagorinsky_cs, guard=guard,
-                           guard_velocity=(u_in, 0.0, 0.0), storage=storage, offset=offset, checkpoint=checkpoint)
+                           guard_velocity=(u_in, 0.0, 0.0), storage=storage, offset=offset, checkpoint=checkpoint,
+                           adjoint_storage=adjoint_storage)
     cd = drag_series(forces, u_in, area)
     loss = cd[-avg_last:].mean()
     torch.cuda.synchronize()
@@ -183,6 +191,7 @@ def sphere_loss_and_grad(nx, ny, nz, n_steps, center_world, radius=0.5, *, avg_l
     return {"grid": [nx, ny, nz], "steps": n_steps, "storage": storage, "q_mode": q_mode,
             "porosity_mode": porosity_mode, "tau": tau, "radius_world": float(radius), "loss": float(loss.detach()),
             "dloss_dr_world": g, "dloss_dr_lattice": g / (nx / 8.0), "checkpoint": str(checkpoint),
+            "adjoint_storage": adjoint_storage or ("f16" if storage in ("f16", "bf16") else "f32"),
             "fwd_s": t_fwd, "fwd_adj_s": wall, "fwd_adj_mlups": nx * ny * nz * n_steps / wall / 1e6}
 
 
@@ -215,6 +224,7 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--storage", default="f32")
     ap.add_argument("--checkpoint", default=None)
+    ap.add_argument("--adjoint-storage", default=None, choices=["f32", "f16"])
     ap.add_argument("--mode", default="hard")
     args = ap.parse_args(argv)
     if args.config == "c1":
@@ -283,7 +293,8 @@ def main(argv=None):
             parts = [int(v) for v in args.checkpoint.split(",")]
             ck = parts[0] if len(parts) == 1 else tuple(parts)
         res = sphere_loss_and_grad(nx, ny, nz, steps, (0.0, 0.0, 0.0), 0.5, storage=args.storage,
-                                   checkpoint=ck, avg_last=int(0.3 * steps), guard="pre")
+                                   checkpoint=ck, avg_last=int(0.3 * steps), guard="pre",
+                                   adjoint_storage=args.adjoint_storage)
         res["config"] = "c4"
         _emit(res)
     elif args.config == "c5":

@@ -35,6 +35,17 @@ class _CountingLattice:
         self.steps += 1
         dst.copy_(src + 1)
 
+    def chain(self, states, fpart=None):
+        for k in range(len(states) - 1):
+            self.step(states[k], states[k + 1])
+
+    def pingpong(self, cur, spare_a, spare_b, n, fpart=None):
+        first = spare_a if cur is not spare_a else spare_b
+        second = spare_b if first is spare_a else spare_a
+        seq = [cur] + [first if k % 2 == 0 else second for k in range(n)]
+        self.chain(seq)
+        return seq[-1]
+
 
 @pytest.mark.parametrize("policy,n", [("all", 7), (3, 10), (4, 4), ((7, 3), 23), ((200, 14), 431), ((5, 5), 12)])
 def test_checkpoint_schedules(policy, n):

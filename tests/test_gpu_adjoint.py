@@ -197,3 +197,56 @@ def test_reduced_precision_gradient_close_to_f64(dev, storage, tol):
         (g,) = torch.autograd.grad(cd, r)
         res[st] = (float(cd), float(g))
     assert abs(res[storage][1] - res["f64"][1]) <= tol * abs(res["f64"][1])
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_hard_mask_equals_binary_porosity_blend(dev, storage):
+    """The soft blend fed porosity = solid in {0, 1} (examples/11:62-65) and the
+    hard-mask reversal give the same forces bit for bit and the same dF/dq."""
+    nx, ny, nz = 32, 16, 16
+    dt = torch.float32
+    r = torch.tensor(0.5, dtype=dt, device=dev)
+    _, solid, q = lbm.smooth_sphere_geometry(nx, ny, nz, (-1.0, 0.0, 0.0), r, 20.0, out_dtype=dt, device=dev)
+    uu = torch.zeros((3, nx, ny, nz), dtype=dt, device=dev)
+    uu[0] = 0.05
+    f0 = lbm.equilibrium(torch.ones((nx, ny, nz), dtype=dt, device=dev), uu)
+    out = []
+    for mode in ("blend", "mask"):
+        qq = q.detach().clone().requires_grad_(True)
+        kw = dict(porosity=solid.to(dt)) if mode == "blend" else dict(solid=solid)
+        s, p, F = A.rollout(f0, 30, tau=0.55, q=qq, collision="mrt", guard="pre", storage=storage,
+                            offset="inlet", **kw)
+        (g,) = torch.autograd.grad(F[-10:, 0].sum(), qq)
+        out.append((cpu(s), cpu(F), cpu(g)))
+    assert torch.equal(out[0][0], out[1][0])
+    assert torch.equal(out[0][1], out[1][1])
+    assert float((out[0][2] - out[1][2]).abs().max()) <= 1e-6 * float(out[1][2].abs().max())
+
+
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_half_cotangent_storage_close_to_float(dev, storage):
+    """Scaled IEEE-half cotangents give the float-cotangent gradient to the
+    half rounding level, and the loss is unchanged."""
+    nx, ny, nz = 32, 16, 16
+    dt = torch.float32
+    out = {}
+    for lam in ("f32", "f16"):
+        r = torch.tensor(0.5, dtype=dt, device=dev, requires_grad=True)
+        _, solid, q = lbm.smooth_sphere_geometry(nx, ny, nz, (-1.0, 0.0, 0.0), r, 20.0, out_dtype=dt, device=dev)
+        uu = torch.zeros((3, nx, ny, nz), dtype=dt, device=dev)
+        uu[0] = 0.05
+        f0 = lbm.equilibrium(torch.ones((nx, ny, nz), dtype=dt, device=dev), uu)
+        _, _, F = A.rollout(f0, 80, tau=0.55, q=q, solid=solid, collision="mrt", guard="pre", storage=storage,
+                            offset="inlet", checkpoint=16, adjoint_storage=lam)
+        cd = A.drag_series(F, 0.05, torch.clamp(lbm.projected_area(solid).to(torch.float64), min=1.0))[-30:].mean()
+        (g,) = torch.autograd.grad(cd, r)
+        out[lam] = (float(cd), float(g))
+    assert out["f16"][0] == out["f32"][0]
+    assert abs(out["f16"][1] - out["f32"][1]) <= 2e-3 * abs(out["f32"][1])
+
+
+def test_half_cotangent_storage_rejected_for_f64(dev):
+    f0 = lbm.equilibrium(torch.ones((8, 6, 4), dtype=torch.float64, device=dev),
+                         torch.zeros((3, 8, 6, 4), dtype=torch.float64, device=dev))
+    with pytest.raises(ValueError):
+        A.rollout(f0, 3, tau=0.6, storage="f64", adjoint_storage="f16")
