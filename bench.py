@@ -9,8 +9,11 @@ per-step float64 momentum-exchange partials.  ``value`` is forward MLUPS
 
 Also reported: the roofline of the dominant kernel (algorithmic 152 B per
 cell update for float32 storage, 76 B for half storage) from HIP events
-around each launch, half-storage forward MLUPS, forward+adjoint MLUPS, and
-the CPU oracle (torch, float32) timed on the host over a bounded sample.
+around each launch, half-storage forward MLUPS, forward+adjoint MLUPS
+(store-all), the dCd/dr pipeline of the 256x128x128 precision study
+(checkpoints every 100 steps) in float32, float16 and float64 storage with
+its wall-time ratio and errors against float64, and the CPU oracle (torch,
+float32) timed on the host over a bounded sample.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--skip-cpu]
 """
@@ -26,6 +29,11 @@ import time
 import torch
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the float32 fused step on this workload, from the
+# FETCH_SIZE / WRITE_SIZE counter passes scaled by a 1 GiB copy
+# (profiles/k_step_mrt_f32_256x128x128_hbm_traffic.md); re-measure when the
+# kernel changes
+MEASURED_TRAFFIC_BYTES = 652.1e6
 NX, NY, NZ = 256, 128, 128
 
 
@@ -114,6 +122,31 @@ def time_forward_adjoint(storage, steps, device, checkpoint="all"):
     return e0.elapsed_time(e1) * 1e-3, float(g), float(loss)
 
 
+def dcd_dr_study(steps, device):
+    """dCd/dr of the time-averaged Cd (last 30% of `steps`) for the 256x128x128
+    sphere with checkpoints every 100 steps, in float32, float16 (half
+    cotangents) and float64 storage: wall time and errors against float64."""
+    from lbm_jax_autodiff_amd.protocols import sphere_loss_and_grad
+    res = {}
+    torch.cuda.empty_cache()
+    for st in ("f32", "f16", "f64"):
+        # two identical runs; the second (allocator warm) is reported
+        for _ in range(2):
+            r = sphere_loss_and_grad(NX, NY, NZ, steps, (0.0, 0.0, 0.0), 0.5, storage=st, checkpoint=100,
+                                     avg_last=int(0.3 * steps), guard="pre")
+        res[st] = r
+        log(f"dCd/dr study {st}: {r['fwd_adj_s']:.2f} s, Cd={r['loss']:.7f}, dCd/dr={r['dloss_dr_world']:.7f}")
+    ref = res["f64"]
+    out = {"steps": steps, "checkpoint": 100, "timing": "second of two identical runs"}
+    for st in ("f32", "f16", "f64"):
+        r = res[st]
+        out[st] = {"s": round(r["fwd_adj_s"], 3), "cd": r["loss"], "dcd_dr": r["dloss_dr_world"],
+                   "cd_rel_err": abs(r["loss"] - ref["loss"]) / abs(ref["loss"]),
+                   "dcd_dr_rel_err": abs(r["dloss_dr_world"] - ref["dloss_dr_world"]) / abs(ref["dloss_dr_world"])}
+    out["f16_speedup_vs_f32"] = round(res["f32"]["fwd_adj_s"] / res["f16"]["fwd_adj_s"], 3)
+    return out
+
+
 def cpu_baseline(sample_steps=2):
     """Oracle (torch CPU float32, the reference's vectorised op structure) on
     the same workload, timed on the host cores over a few steps."""
@@ -165,6 +198,8 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--adj-steps", type=int, default=200)
+    ap.add_argument("--study-steps", type=int, default=2800,
+                    help="steps of the dCd/dr precision study (0 skips it; the full study is 28000)")
     ap.add_argument("--no-extras", action="store_true", help="forward float32 only")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (known bytes) for hardware-counter scaling")
@@ -211,7 +246,7 @@ def main():
             del lat_h
         # forward + hand-written adjoint, store-all, dCd/dr of the time-averaged Cd
         adj_steps = args.adj_steps
-        for st, bpl in (("f32", 380.0), ("f16", 266.0)):
+        for st, bpl in (("f32", 380.0), ("f16", 190.0)):
             time_forward_adjoint(st, 8, dev)  # warm-up (plans, allocator)
             torch.cuda.empty_cache()
             sec, g, loss = time_forward_adjoint(st, adj_steps, dev)
@@ -222,7 +257,11 @@ def main():
             log(f"{st} fwd+adj ({adj_steps} steps, store-all): {sec:.3f} s, {ml:.1f} MLUPS, dCd/dr={g:.6g}")
             torch.cuda.empty_cache()
         extras["fwd_adj_steps"] = adj_steps
-        extras["fwd_adj_bytes_per_cell_update"] = {"f32": 380.0, "f16": 266.0}
+        # forward 152 / 76 B plus adjoint: P_t, lambda in, lambda out (float32
+        # cotangents for f32 storage, scaled half cotangents for f16)
+        extras["fwd_adj_bytes_per_cell_update"] = {"f32": 380.0, "f16": 190.0}
+        if args.study_steps > 0:
+            extras["dcd_dr_study"] = dcd_dr_study(args.study_steps, dev)
 
     out = {
         "metric": "fwd & fwd+adjoint MLUPS, % HBM roofline; dCd/dr time at <=1e-3 rel err",
@@ -241,7 +280,7 @@ def main():
                                "fused forward step", "grid": [NX, NY, NZ], "tau": round(tau, 6),
                    "cells": cells, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": MEASURED_TRAFFIC_BYTES,
                      "kernel_ms": round(k_ms, 5), "bytes_per_cell_update": bytes_per_lu},
         "extras": extras,
     }
