@@ -1,7 +1,6 @@
 // Hand-written adjoint of the fused D3Q19 step (reverse mode over one step).
 //
-// Given P_t and lambda_{t+1} = dL/dP_{t+1}, one step (a launch for the link
-// cells, then one for the dense grid) recomputes each cell's
+// Given P_t and lambda_{t+1} = dL/dP_{t+1}, one launch recomputes each cell's
 // forward step (pull, Bouzidi, Zou-He, guards, collision, bounce) and pulls
 // the cotangent back to mu = dL/d(pulled values).  Pull streaming is a
 // bijection once the x faces and the y/z axes wrap: entry (y, i) of P_t is
@@ -19,7 +18,6 @@
 namespace lbm {
 
 constexpr int kAdjBlock = 256;
-constexpr int kAdjLinkBlock = 64;
 
 template <typename R>
 struct AdjArgs {
@@ -98,50 +96,73 @@ __device__ __forceinline__ bool x_fallback(int x, int nx) {
   else return false;
 }
 
-// Pulled values of cell x (normal pulls and, for link cells, Bouzidi).  For
-// link cells qv holds q of the 18 directions (qv[0] unused; loaded by the
-// caller together with the cell index), own receives the cell's
-// P_t and v2[i] the value loaded for direction i (the second neighbour
-// P(x + e_i)_opp where 0 <= q_opp < 1/2); every load is unconditional so a link cell costs three
-// dependent round trips instead of one per direction.
-template <class S, bool LINK>
-__device__ __forceinline__ void pull_cell(const AdjArgs<typename S::R>& a, const typename S::T* __restrict__ P,
-                                          uint32_t c, int x, int y, int z, int b, typename S::R* s,
-                                          typename S::R* own, const typename S::R* qv, typename S::R* v2) {
-  using R = typename S::R;
+// Pulled values of a non-link cell x (streaming.py:113-142): x - e_i with all
+// axes wrapped, the cell itself at the x faces.
+template <class S>
+__device__ __forceinline__ void pull_dense(const AdjArgs<typename S::R>& a, const typename S::T* __restrict__ P,
+                                           uint32_t c, int x, int y, int z, typename S::R* s) {
   const size_t N = a.P;
-  if constexpr (LINK) {
-    sfor<Q>([&](auto I) {
-      constexpr int i = I;
-      own[i] = S::ld_c(P, i * N + c, a.off[i]);
-    });
-    s[0] = own[0];
-  } else {
-    s[0] = S::ld_c(P, c, a.off[0]);
-  }
+  s[0] = S::ld_c(P, c, a.off[0]);
   sfor<Q>([&](auto I) {
     constexpr int i = I;
     if constexpr (i != 0) {
-      constexpr int io = opp(i);
-      const size_t normal = i * N + (x_fallback<i>(x, a.nx) ? c : owner_target<i>(a, x, y, z));
-      if constexpr (LINK) {
+      const size_t k = i * N + (x_fallback<i>(x, a.nx) ? c : owner_target<i>(a, x, y, z));
+      s[i] = S::ld_c(P, k, a.off[i]);
+    }
+  });
+}
+
+// Pulled values of a link cell with Bouzidi (streaming.py:113-181), and
+// dS_i/dq of every link direction into dsdq (0 elsewhere and outside the
+// clip bounds).  qv holds q of the 18 directions.  The directions are
+// processed in chunks of CH, each issuing its loads together (the cell's own
+// P_opp(i), and the second neighbour P(x + e_i)_opp, the own P_i or the normal
+// pull); all loads are unconditional, so the cell costs 18/CH dependent
+// round trips and keeps no per-direction state in registers afterwards.
+template <class S, int CH>
+__device__ __forceinline__ void pull_link(const AdjArgs<typename S::R>& a, const typename S::T* __restrict__ P,
+                                          uint32_t c, int x, int y, int z, const typename S::R* qv,
+                                          typename S::R* s, typename S::R* dsdq) {
+  using R = typename S::R;
+  const size_t N = a.P;
+  s[0] = S::ld_c(P, c, a.off[0]);
+  sfor<(Q - 1 + CH - 1) / CH>([&](auto K) {
+    constexpr int k0 = 1 + int(K) * CH;
+    R va[CH], vb[CH];
+    sfor<CH>([&](auto J) {
+      constexpr int i = k0 + int(J);
+      if constexpr (i < Q) {
+        constexpr int io = opp(i);
         const R qo = qv[io];
         const bool low = (qo >= R(0)) & (qo < R(0.5));
-        const R v = S::ld_c(P, low ? io * N + next_cell<i>(a, x, y, z) : normal, low ? a.off[io] : a.off[i]);
+        const bool high = qo >= R(0.5);
+        const size_t normal = i * N + (x_fallback<i>(x, a.nx) ? c : owner_target<i>(a, x, y, z));
+        const size_t ka = low ? io * N + next_cell<i>(a, x, y, z) : (high ? i * N + c : normal);
+        va[J] = S::ld_c(P, ka, low ? a.off[io] : a.off[i]);
+        vb[J] = S::ld_c(P, io * N + c, a.off[io]);
+      }
+    });
+    sfor<CH>([&](auto J) {
+      constexpr int i = k0 + int(J);
+      if constexpr (i < Q) {
+        constexpr int io = opp(i);
+        const R qo = qv[io];
+        R d = R(0);
         if (qo >= R(0.5)) {
           const R inv2q = R(1) / (R(2) * fmin(qo, R(1)));
-          s[i] = inv2q * own[io] + (R(1) - inv2q) * own[i];
-        } else if (low) {
+          s[i] = inv2q * vb[J] + (R(1) - inv2q) * va[J];
+          if (qo <= R(1)) d = -(vb[J] - va[J]) / (R(2) * qo * qo);
+        } else if (qo >= R(0)) {
           const R twoq = R(2) * qo;
-          s[i] = twoq * own[io] + (R(1) - twoq) * v;
+          s[i] = twoq * vb[J] + (R(1) - twoq) * va[J];
+          d = R(2) * (vb[J] - va[J]);
         } else {
-          s[i] = v;
+          s[i] = va[J];
         }
-        v2[i] = v;
-      } else {
-        s[i] = S::ld_c(P, normal, a.off[i]);
+        dsdq[i] = d;
       }
-    }
+    });
+    __builtin_amdgcn_sched_barrier(0);
   });
 }
 
@@ -198,7 +219,8 @@ template <class S, class LM, int COLL, int MODE, bool LINK>
 __device__ __forceinline__ void adjoint_cell(const AdjArgs<typename S::R>& a, const typename S::T* __restrict__ P,
                                              const typename LM::T* __restrict__ lam_in,
                                              typename LM::T* __restrict__ lam_out, const double* __restrict__ gF,
-                                             uint32_t c, int b, uint32_t fl, const typename S::R* qv) {
+                                             uint32_t c, int b, uint32_t fl, const typename S::R* qv,
+                                             typename S::R* dsdq) {
   using R = typename S::R;
   const size_t N = a.P;
   // a dense-sweep thread on a link cell computes but writes nothing (the
@@ -206,8 +228,17 @@ __device__ __forceinline__ void adjoint_cell(const AdjArgs<typename S::R>& a, co
   const bool skip = !LINK && (fl & 2u);
   int x, y, z;
   adj_coords(a, c, x, y, z);
-  R s[Q], own[LINK ? Q : 1], v2[LINK ? Q : 1];
-  pull_cell<S, LINK>(a, P, c, x, y, z, b, s, own, qv, v2);
+  R s[Q];
+  uint32_t lmask = 0;  // bit j: direction j is a wall link (q_j >= 0)
+  if constexpr (LINK) {
+    pull_link<S, 6>(a, P, c, x, y, z, qv, s, dsdq);
+    sfor<Q>([&](auto J) {
+      constexpr int j = J;
+      if constexpr (j != 0) lmask |= (qv[j] >= R(0)) ? (1u << j) : 0u;
+    });
+  } else {
+    pull_dense<S>(a, P, c, x, y, z, s);
+  }
   // the inlet's density depends on the pulled values it keeps
   R rho_in = R(0);
   if (x == 0) rho_in = zou_he_in_rho(s, a.uin[0]);
@@ -227,7 +258,7 @@ __device__ __forceinline__ void adjoint_cell(const AdjArgs<typename S::R>& a, co
       sfor<Q>([&](auto J) {
         constexpr int j = J;
         if constexpr (j != 0) {
-          if (qv[j] >= R(0)) {
+          if (lmask & (1u << j)) {
             R v = R(0);
             add_scaled<EX[j]>(v, gx);
             add_scaled<EY[j]>(v, gy);
@@ -263,51 +294,16 @@ __device__ __forceinline__ void adjoint_cell(const AdjArgs<typename S::R>& a, co
     constexpr int i = I;
     if constexpr (i != 0) {
       constexpr int io = opp(i);
-      bool link = false;
-      R qo = R(-1);
-      if constexpr (LINK) {
-        qo = qv[io];
-        link = qo >= R(0);
-      }
+      const bool link = LINK && (lmask & (1u << io));
       const bool normal = !link && !x_fallback<i>(x, a.nx);
       LM::st(lam_out, i * N + owner_target<i>(a, x, y, z), normal ? g[i] : R(0), a.lam_scale);
       if constexpr (LINK) {
         a.mu_link[size_t(i) * a.n_links + b] = link ? g[i] : R(0);
-        if (link && a.dq_acc) {
-          // d S_i / d q on the taken branch; 0 outside the clip bounds
-          R dsdq = R(0);
-          if (qo >= R(0.5)) {
-            if (qo <= R(1)) dsdq = -(own[io] - own[i]) / (R(2) * qo * qo);
-          } else {
-            dsdq = R(2) * (own[io] - v2[i]);
-          }
-          a.dq_acc[size_t(io - 1) * a.n_links + b] += double(g[i]) * double(dsdq);
-        }
+        // d S_i / d q_opp(i) on the taken branch (from the pull)
+        if (link && a.dq_acc) a.dq_acc[size_t(io - 1) * a.n_links + b] += double(g[i]) * double(dsdq[i]);
       }
     }
   });
-}
-
-// Link cells (Bouzidi transpose, mu_link, dq): a small launch ahead of the
-// dense sweep, as in the forward step.
-template <class S, class LM, int COLL, int MODE>
-__global__ __launch_bounds__(kAdjLinkBlock) void k_adjoint_links(AdjArgs<typename S::R> a,
-                                                                 const typename S::T* __restrict__ P,
-                                                                 const typename LM::T* __restrict__ lam_in,
-                                                                 typename LM::T* __restrict__ lam_out,
-                                                                 const double* __restrict__ gF) {
-  using R = typename S::R;
-  const int b = int(blockIdx.x) * kAdjLinkBlock + int(threadIdx.x);
-  if (b >= a.n_links) return;
-  // the cell index and its q values in one round trip
-  const uint32_t c = uint32_t(a.link_cell[b]);
-  R qv[Q];
-  qv[0] = R(-1);
-  sfor<Q>([&](auto I) {
-    constexpr int i = I;
-    if constexpr (i != 0) qv[i] = R(a.link_q[size_t(i - 1) * a.n_links + b]);
-  });
-  adjoint_cell<S, LM, COLL, MODE, true>(a, P, lam_in, lam_out, gF, c, b, a.flags[c], qv);
 }
 
 // Occupancy target of the dense adjoint sweep: the FP32 MRT transpose fits
@@ -318,16 +314,39 @@ constexpr int adj_waves_per_eu() {
   return (sizeof(typename S::R) == 4 && COLL == LBM_COLL_MRT) ? 4 : 1;
 }
 
-// Dense sweep; link cells are recomputed but written by k_adjoint_links.
+// One adjoint step: workgroups [0, nlb) own the link cells (Bouzidi transpose,
+// mu_link, dq), the rest sweep the dense grid; dense threads on link cells
+// compute but write nothing.
 template <class S, class LM, int COLL, int MODE>
 __global__ __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(adj_waves_per_eu<S, COLL>()))) void k_adjoint(
     AdjArgs<typename S::R> a, const typename S::T* __restrict__ P, const typename LM::T* __restrict__ lam_in,
     typename LM::T* __restrict__ lam_out, const double* __restrict__ gF) {
-  const uint32_t tile = xcd_swizzle(blockIdx.x, uint32_t(a.ndb));
+  using R = typename S::R;
+  const uint32_t bid = blockIdx.x;
+  if (int(bid) < a.nlb) {
+    // per-thread dS/dq of the link directions, kept out of the registers
+    __shared__ R dsdq[kAdjBlock][Q];
+    const int b = int(bid) * kAdjBlock + int(threadIdx.x);
+    if (b >= a.n_links) return;
+    const uint32_t c = uint32_t(a.link_cell[b]);
+    R qv[Q];
+    qv[0] = R(-1);
+    sfor<Q>([&](auto I) {
+      constexpr int i = I;
+      if constexpr (i != 0) qv[i] = R(a.link_q[size_t(i - 1) * a.n_links + b]);
+    });
+    // pin the converted q here (a conversion sunk to its uses keeps the
+    // float64 values alive)
+#pragma unroll
+    for (int i = 1; i < Q; ++i) asm volatile("" : "+v"(qv[i]));
+    adjoint_cell<S, LM, COLL, MODE, true>(a, P, lam_in, lam_out, gF, c, b, a.flags[c], qv, dsdq[threadIdx.x]);
+    return;
+  }
+  const uint32_t tile = xcd_swizzle(bid - uint32_t(a.nlb), uint32_t(a.ndb));
   const uint32_t c = tile * kAdjBlock + threadIdx.x;
   if (c >= a.N) return;
   adjoint_cell<S, LM, COLL, MODE, false>(a, P, lam_in, lam_out, gF, c, -1, a.flags ? uint32_t(a.flags[c]) : 0u,
-                                     nullptr);
+                                         nullptr, nullptr);
 }
 
 // Prologue transpose: dL/dS_0 = ([pre guard] collide bounce)^T lambda_0, cell-local.
@@ -422,7 +441,7 @@ int make_adj_args(const lbm_step_desc* d, const lbm_adjoint_desc* ad, int mode, 
   if (a.bounce == LBM_BOUNCE_HARD && !a.flags) return LBM_E_ARG;
   if (a.bounce == LBM_BOUNCE_SOFT && !a.phi) return LBM_E_ARG;
   if (a.n_links < 0 || (a.n_links > 0 && (!a.link_cell || !a.link_q || !a.flags || !a.mu_link))) return LBM_E_ARG;
-  a.nlb = (mode == LBM_MODE_COLLIDE) ? 0 : (a.n_links + kAdjLinkBlock - 1) / kAdjLinkBlock;
+  a.nlb = (mode == LBM_MODE_COLLIDE) ? 0 : (a.n_links + kAdjBlock - 1) / kAdjBlock;
   a.ndb = int((n + kAdjBlock - 1) / kAdjBlock);
   return LBM_OK;
 }
@@ -446,22 +465,9 @@ int launch_adjoint(const lbm_step_desc* d, const lbm_adjoint_desc* ad, int mode,
     else hipLaunchKernelGGL((k_adjoint_collide<S, LM, LBM_COLL_BGK>), grid, kAdjBlock, 0, st, a, P, li, go);
   } else {
     auto* lo = static_cast<typename LM::T*>(lam_out);
-    const unsigned grid = unsigned(a.ndb);
-#define LBM_ADJ(COLL, MODE)                                                                                     \
-  do {                                                                                                          \
-    auto dense = [&](hipStream_t q) {                                                                           \
-      hipLaunchKernelGGL((k_adjoint<S, LM, COLL, MODE>), grid, kAdjBlock, 0, q, a, P, li, lo, gF);                  \
-    };                                                                                                          \
-    if (a.nlb > 0) {                                                                                            \
-      auto links = [&](hipStream_t q) {                                                                         \
-        hipLaunchKernelGGL((k_adjoint_links<S, LM, COLL, MODE>), unsigned(a.nlb), kAdjLinkBlock, 0, q, a, P, li, lo, gF); \
-      };                                                                                                        \
-      links(st);                                                                                                \
-      dense(st);                                                                                                \
-    } else {                                                                                                    \
-      dense(st);                                                                                                \
-    }                                                                                                           \
-  } while (0)
+    const unsigned grid = unsigned(a.nlb + a.ndb);
+#define LBM_ADJ(COLL, MODE) \
+  hipLaunchKernelGGL((k_adjoint<S, LM, COLL, MODE>), grid, kAdjBlock, 0, st, a, P, li, lo, gF)
     if (mode == LBM_MODE_FULL) {
       if (mrt) LBM_ADJ(LBM_COLL_MRT, LBM_MODE_FULL);
       else LBM_ADJ(LBM_COLL_BGK, LBM_MODE_FULL);

@@ -446,17 +446,34 @@ __global__ void k_links_count(const R* __restrict__ q, int32_t* __restrict__ cou
   }
 }
 
-__global__ void k_exclusive_scan(const int32_t* __restrict__ counts, int32_t* __restrict__ offsets, int n,
-                                 int32_t* __restrict__ total) {
-  // single thread: the block count is at most a few hundred thousand and this
-  // runs once per geometry
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+constexpr int kScanThreads = 1024;
+
+// Exclusive prefix sum of the per-block link counts in one workgroup: each
+// thread sums a contiguous chunk, the chunk totals are scanned in LDS, then
+// each thread writes its chunk's offsets.  Runs once per geometry.
+__global__ __launch_bounds__(kScanThreads) void k_exclusive_scan(const int32_t* __restrict__ counts,
+                                                                 int32_t* __restrict__ offsets, int n,
+                                                                 int32_t* __restrict__ total) {
+  __shared__ int32_t part[kScanThreads];
+  const int t = int(threadIdx.x);
+  const int per = (n + kScanThreads - 1) / kScanThreads;
+  const int b0 = min(n, t * per), b1 = min(n, b0 + per);
   int32_t s = 0;
-  for (int b = 0; b < n; ++b) {
-    offsets[b] = s;
-    s += counts[b];
+  for (int b = b0; b < b1; ++b) s += counts[b];
+  part[t] = s;
+  __syncthreads();
+  for (int off = 1; off < kScanThreads; off <<= 1) {
+    const int32_t v = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
   }
-  *total = s;
+  int32_t run = t > 0 ? part[t - 1] : 0;
+  for (int b = b0; b < b1; ++b) {
+    offsets[b] = run;
+    run += counts[b];
+  }
+  if (t == kScanThreads - 1) *total = part[t];
 }
 
 template <typename R>
@@ -715,7 +732,8 @@ int lbm_links_count(int dtype, const void* q, int32_t* block_counts, int32_t* bl
   const unsigned nb = grid_for(N);
   LBM_DISPATCH_RF(dtype, hipLaunchKernelGGL(k_links_count<R>, nb, kBlock, 0, (hipStream_t)stream,
                                             (const R*)q, block_counts, N));
-  hipLaunchKernelGGL(k_exclusive_scan, 1, 1, 0, (hipStream_t)stream, block_counts, block_offsets, int(nb), total);
+  hipLaunchKernelGGL(k_exclusive_scan, 1, kScanThreads, 0, (hipStream_t)stream, block_counts, block_offsets, int(nb),
+                     total);
   return launch_status();
 }
 

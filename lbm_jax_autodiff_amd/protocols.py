@@ -167,6 +167,10 @@ def sphere_loss_and_grad(nx, ny, nz, n_steps, center_world, radius=0.5, *, avg_l
     # float64 dL/dp accumulation.
     porosity = por_soft if porosity_mode == "soft" else None
     hard_solid = solid if porosity_mode == "hard" else None
+    if q_mode == "halfway" and porosity_mode == "hard":
+        # nothing else depends on r here: keep the reference's 0 * porosity(r)
+        # term so the adjoint runs and returns its exact zero
+        porosity, hard_solid = solid.to(dt) + 0.0 * por_soft, None
     D = 2.0 * float(r_ref) * nx / 8.0
     tau = tau_for(u_in, D, re)
     u = torch.zeros((3, nx, ny, nz), dtype=dt, device=dev)
@@ -288,7 +292,8 @@ def main(argv=None):
     elif args.config == "c4":
         nx, ny, nz = 256, 128, 128
         steps = args.steps or 28000
-        ck = 100
+        # every 100 steps; float64 states (610 MiB each) take two levels
+        ck = (200, 14) if args.storage == "f64" else 100
         if args.checkpoint:
             parts = [int(v) for v in args.checkpoint.split(",")]
             ck = parts[0] if len(parts) == 1 else tuple(parts)
