@@ -167,6 +167,11 @@ class _Checkpoints:
     def _seg(self, have, t0, t1, level):
         if self.levels[level] == 1:
             yield t0, t1, have
+            if level > 0:
+                # a caller's loop variable still names this dict while the
+                # next segment is recomputed: release its states here, or
+                # two segments' worth would be alive at once
+                have.clear()
             return
         starts = sorted(t for t in have if t0 <= t < t1)
         ends = starts[1:] + [t1]

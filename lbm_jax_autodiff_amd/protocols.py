@@ -153,6 +153,7 @@ def sphere_loss_and_grad(nx, ny, nz, n_steps, center_world, radius=0.5, *, avg_l
     viscosity comes from the nominal diameter 2 r_ref (examples/11:228-230),
     so tau does not move with r."""
     dev = _dev()
+    torch.cuda.reset_peak_memory_stats(dev)
     dt = _dtype(storage)
     gdt = geometry_dtype or (torch.float64 if storage == "f64" else torch.float32)
     r = torch.tensor(float(radius), dtype=gdt, device=dev, requires_grad=True)
@@ -195,8 +196,9 @@ def sphere_loss_and_grad(nx, ny, nz, n_steps, center_world, radius=0.5, *, avg_l
     return {"grid": [nx, ny, nz], "steps": n_steps, "storage": storage, "q_mode": q_mode,
             "porosity_mode": porosity_mode, "tau": tau, "radius_world": float(radius), "loss": float(loss.detach()),
             "dloss_dr_world": g, "dloss_dr_lattice": g / (nx / 8.0), "checkpoint": str(checkpoint),
-            "adjoint_storage": adjoint_storage or ("f16" if storage in ("f16", "bf16") else "f32"),
-            "fwd_s": t_fwd, "fwd_adj_s": wall, "fwd_adj_mlups": nx * ny * nz * n_steps / wall / 1e6}
+            "adjoint_storage": adjoint_storage or ("f16" if storage in ("f16", "bf16") else storage),
+            "fwd_s": t_fwd, "fwd_adj_s": wall, "fwd_adj_mlups": nx * ny * nz * n_steps / wall / 1e6,
+            "peak_hbm_gib": torch.cuda.max_memory_allocated(dev) / 2.0 ** 30}
 
 
 def adam(loss_and_grad, r0, n_iters, lr, r_min=0.3, r_max=0.8, b1=0.9, b2=0.999, eps=1e-8):
@@ -306,8 +308,14 @@ def main(argv=None):
         nx, ny, nz = 384, 192, 192
         steps = args.steps or 40000
         soft = args.mode == "soft"
+        # segments of 200 steps with sub-checkpoints every 14 for half storage
+        # (123 GB); fewer, longer segments for the 1.08 GB float32 and 2.15 GB
+        # float64 states: about N/K1 + K1/K2 + K2 states are held at once
+        ck = {"f32": (400, 20), "f64": (1200, 35)}.get(args.storage, (200, 14))
+        if args.checkpoint:
+            ck = tuple(int(v) for v in args.checkpoint.split(","))
         res = sphere_loss_and_grad(nx, ny, nz, steps, (0.0, 0.0, 0.0), 0.5, storage=args.storage,
-                                   checkpoint=(200, 14), avg_last=int(0.3 * steps), guard="pre",
+                                   checkpoint=ck, avg_last=int(0.3 * steps), guard="pre",
                                    porosity_mode="soft" if soft else "hard", sharpness=10.0,
                                    smagorinsky_cs=0.1 if soft else None)
         res["config"] = "c5-" + args.mode

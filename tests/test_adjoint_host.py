@@ -9,6 +9,7 @@
 """
 
 import types
+import weakref
 
 import numpy as np
 import pytest
@@ -63,6 +64,59 @@ def test_checkpoint_schedules(policy, n):
         assert float(p) == float(t)
         seen.append(t)
     assert seen == list(range(n - 1, -1, -1))
+
+
+class _TrackingLattice(_CountingLattice):
+    """Counting lattice that also tracks how many state buffers are alive."""
+
+    def __init__(self):
+        super().__init__()
+        self.refs = []
+        self.peak = 0
+
+    def _track(self, t):
+        self.refs.append(weakref.ref(t))
+        self.peak = max(self.peak, self.live())
+        return t
+
+    def live(self):
+        return sum(1 for r in self.refs if r() is not None)
+
+    def new_state(self):
+        return self._track(torch.zeros(1))
+
+    def clone_state(self, buf):
+        return self._track(buf.clone())
+
+
+@pytest.mark.parametrize("policy,n", [(100, 2800), ((200, 14), 4000), ((600, 25), 12000)])
+def test_checkpoint_peak_states(policy, n):
+    """At most about N/K1 + K1/K2 + K2 states are alive while the backward
+    walks the segments, with the consumer's loop variable still holding the
+    previous segment (as the rollout's backward loop does)."""
+    lat = _TrackingLattice()
+    ck = A._Checkpoints(lat, policy, n)
+    a, b = lat.new_state(), lat.new_state()
+    cur, t = a, 0
+    while t < n - 1:
+        ck.record(t, cur)
+        end = min(n - 1, (t // ck.stride + 1) * ck.stride)
+        cur = lat.pingpong(cur, a, b, end - t)
+        t = end
+    ck.record(n - 1, cur)
+    expect_t = n - 1
+    for t0, t1, have in ck.segments():
+        for t in range(t1 - 1, t0 - 1, -1):
+            assert float(have[t]) == float(t) and t == expect_t
+            expect_t -= 1
+    assert expect_t == -1
+    if isinstance(policy, int):
+        bound = -(-n // policy) + policy
+    else:
+        k1, k2 = policy
+        bound = -(-n // k1) + -(-k1 // k2) + k2
+    # + the two ping-pong buffers of each recompute level and the forward's pair
+    assert lat.peak <= bound + 6, (lat.peak, bound)
 
 
 def _owner_transpose(mu, q, nx, ny, nz):

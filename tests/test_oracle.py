@@ -2,12 +2,18 @@
 planted-mistake checks (each plants an error in the oracle and shows a pin
 catches it)."""
 
+import json
 import math
+import os
 
 import pytest
 import torch
 
 from oracle import lbm_oracle as O
+
+# values the reference publishes for this path, with their sources
+with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "kats.json")) as _fh:
+    KAT = json.load(_fh)
 
 
 # ---------------------------------------------------------------- lattice --
@@ -65,7 +71,7 @@ def test_kat_equilibrium_mass_gradient():
     rho = torch.tensor(1.0, dtype=torch.float64, requires_grad=True)
     f = O.equilibrium(rho.expand(4, 4, 4), torch.zeros((3, 4, 4, 4), dtype=torch.float64))
     (g,) = torch.autograd.grad(f.sum(), rho)
-    assert abs(float(g) - 64.0) < 1e-5
+    assert abs(float(g) - KAT["d_sum_feq_d_rho"]["value"]) < 1e-5
 
 
 # ------------------------------------------------------------ collision --
@@ -148,13 +154,17 @@ def test_zou_he_post_conditions():
 # -------------------------------------------------------------- forces --
 
 def test_forces_known_values():
-    assert abs(float(O.drag_coefficient(torch.tensor([0.01, 0.0, 0.0]), 0.05, 100.0)) - 0.08) < 1e-6
-    solid = torch.zeros((16, 16, 16), dtype=torch.int32)
-    solid[6:10, 6:10, 6:10] = 1
-    assert float(O.projected_area(solid, 0)) == 16.0
-    ground = torch.zeros((8, 8, 8), dtype=torch.int32)
-    ground[:, :, 0] = 2
-    assert float(O.projected_area(ground, 0)) == 0.0
+    k = KAT["drag_coefficient"]
+    cd = O.drag_coefficient(torch.tensor([k["force_x"], 0.0, 0.0]), k["u"], k["area"])
+    assert abs(float(cd) - k["value"]) < 1e-6
+    k = KAT["projected_area_box"]
+    solid = torch.zeros(k["shape"], dtype=torch.int32)
+    (x0, x1), (y0, y1), (z0, z1) = k["box"]
+    solid[x0:x1, y0:y1, z0:z1] = 1
+    assert float(O.projected_area(solid, 0)) == k["value"]
+    ground = torch.zeros((16, 8, 8), dtype=torch.int32)
+    ground[:, 0, :] = 2
+    assert float(O.projected_area(ground, 0)) == KAT["projected_area_ground"]["value"]
     f = torch.rand((19, 4, 4, 4), dtype=torch.float64)
     assert float(O.momentum_exchange(f, f, -torch.ones_like(f)).abs().max()) == 0.0
 
@@ -191,21 +201,20 @@ def test_smooth_solid_matches_create_sphere_without_ties():
 def test_kat_inlet_velocity_gradient():
     """FP64 d mean(rho)/dU over 5 steps = 0.611238 (paper/paper.tex:502)."""
     g, _ = O.kat_inlet_velocity_grad()
-    assert abs(g - 0.611238) < 5e-7
+    assert abs(g - KAT["d_mean_rho_d_u_inlet"]["value"]) < 5e-7
 
 
 def test_kat_bouzidi_radius_gradient():
     """FP64 Bouzidi dCd/dr over 10 steps = 93.33275 (paper/paper.tex:503)."""
     g, _ = O.kat_bouzidi_grad()
-    assert abs(g - 93.33275) < 5e-5
+    assert abs(g - KAT["d_cd_d_r_bouzidi_soft"]["value"]) < 5e-5
 
 
 def test_kat_poiseuille_profile():
     """60x20x4 BGK tau=0.8, 3000 steps, FP32: README.md:59-69 to 6 decimals."""
     prof, _ = O.channel_flow()
-    published = {0: 0.000200, 1: 0.008247, 5: 0.057873, 9: 0.076879, 10: 0.076879, 15: 0.048352, 19: 0.000200}
-    for y, v in published.items():
-        assert round(float(prof[y]), 6) == v, (y, float(prof[y]))
+    for y, v in KAT["poiseuille_60x20x4"]["ux_by_y"].items():
+        assert round(float(prof[int(y)]), 6) == v, (y, float(prof[int(y)]))
 
 
 # ---------------------------------------------------- planted mistakes --
