@@ -231,9 +231,19 @@ typedef struct lbm_step_desc {
                                means 1 */
   int64_t pop_stride;       /* elements between population i and i+1 of every
                                state and cotangent buffer the fused entry
-                               points read or write (the storage is
-                               [19][pop_stride], cells x-fastest); 0 = N.
-                               lbm_state_stride gives the recommended value */
+                               points read or write; 0 = N */
+  int64_t row_stride;       /* elements between consecutive (y, z) rows; 0 = nx.
+                               Population i of cell (x, y, z) is element
+                                 i * pop_stride + (z * ny + y) * row_stride + x.
+                               Two layouts are accepted:
+                               - populations apart: row_stride = nx and
+                                 pop_stride >= N ([19][pop_stride], cells
+                                 x-fastest; lbm_state_stride gives a padded
+                                 pop_stride);
+                               - rows interleaved: pop_stride >= nx and
+                                 row_stride >= 19 * pop_stride ([nz][ny][19][nx]
+                                 for pop_stride = nx, row_stride = 19 nx: the
+                                 19 populations of an x-row are adjacent) */
 } lbm_step_desc;
 
 /* Recommended pop_stride (>= N) for the grid of d: a padded stride that keeps

@@ -42,6 +42,7 @@ class StepDesc(ctypes.Structure):
         ("flags", ctypes.c_void_p), ("porosity", ctypes.c_void_p),
         ("link_cell", ctypes.c_void_p), ("link_q", ctypes.c_void_p),
         ("cells_per_thread", ctypes.c_int32), ("pop_stride", ctypes.c_int64),
+        ("row_stride", ctypes.c_int64),
     ]
 
 

@@ -60,12 +60,14 @@ def scatter_plan(lat: Lattice):
     if n == 0:
         return None
     nx, ny, nz = lat.nx, lat.ny, lat.nz
-    N = getattr(lat, "P", lat.N)  # population stride of the cotangent buffers
+    N = lat.P  # population stride of the cotangent buffers
+    RS = lat.RS  # row stride
     cell = lat.links.cell.detach().to("cpu").numpy().astype(np.int64)
     q = lat.links.q.detach().to("cpu").numpy()  # (18, n)
     x = cell % nx
     y = (cell // nx) % ny
     z = cell // (nx * ny)
+    off = (z * ny + y) * RS + x  # element offset of each link cell in a population
     keys, srcs, coefs = [], [], []
     b = np.arange(n, dtype=np.int64)
     for j in range(1, 19):
@@ -74,7 +76,7 @@ def scatter_plan(lat: Lattice):
         if not has.any():
             continue
         i = _OPP[j]
-        bb, qq, cc = b[has], qj[has], cell[has]
+        bb, qq, cc = b[has], qj[has], off[has]
         src = i * n + bb
         hi = qq >= 0.5
         inv2q = 1.0 / (2.0 * np.clip(qq, 0.5, 1.0))
@@ -92,7 +94,7 @@ def scatter_plan(lat: Lattice):
         fx = (x[has][lo] + _E[i, 0]) % nx
         fy = (y[has][lo] + _E[i, 1]) % ny
         fz = (z[has][lo] + _E[i, 2]) % nz
-        keys.append(j * N + (fz * ny + fy) * nx + fx)
+        keys.append(j * N + (fz * ny + fy) * RS + fx)
         srcs.append(src[lo])
         coefs.append(1.0 - twoq[lo])
         # force term on P_j(x)
@@ -197,7 +199,7 @@ class _RolloutFn(torch.autograd.Function):
                       porosity=porosity.detach() if porosity is not None else None,
                       q=q.detach() if q is not None else None, guard=cfg["guard"],
                       guard_velocity=cfg["guard_velocity"], periodic_yz=True, storage=cfg["storage"],
-                      offset=cfg["offset"], device=f0.device)
+                      offset=cfg["offset"], device=f0.device, layout=cfg["layout"])
         ck = _Checkpoints(lat, cfg["checkpoint"], n_steps)
         s0 = lat.to_state(f0.detach())
         a = lat.new_state()
@@ -347,7 +349,8 @@ def rollout(f0: torch.Tensor, n_steps: int, *, tau, q: Optional[torch.Tensor] = 
             inlet_velocity: Union[Sequence[float], torch.Tensor] = (0.05, 0.0, 0.0), rho_out: float = 1.0,
             collision: str = "bgk", smagorinsky_cs: Optional[float] = None, guard: str = "none",
             guard_velocity=None, storage: str = "f32", offset="weights",
-            checkpoint: Union[str, int, Sequence[int]] = "all", adjoint_storage: Optional[str] = None):
+            checkpoint: Union[str, int, Sequence[int]] = "all", adjoint_storage: Optional[str] = None,
+            layout: str = "rows"):
     """Differentiable fused rollout: (S_N, P_{N-1}, forces[N, 3] float64).
 
     Step order as in the reference's drivers: [pre guard] -> [Smagorinsky]
@@ -355,7 +358,8 @@ def rollout(f0: torch.Tensor, n_steps: int, *, tau, q: Optional[torch.Tensor] = 
     with Bouzidi (q) -> Zou-He inlet/outlet -> [post guard].
     ``adjoint_storage`` "f32" keeps the cotangents in the arithmetic type,
     "f16" stores them as scaled IEEE half (half storages only; default for
-    them), halving the adjoint's cotangent traffic.
+    them), halving the adjoint's cotangent traffic.  ``layout`` places the
+    state and cotangent buffers ("rows" or "apart", as for ``Lattice``).
     """
     nx, ny, nz = F.dims_of_pop(f0)
     if isinstance(inlet_velocity, torch.Tensor):
@@ -369,7 +373,7 @@ def rollout(f0: torch.Tensor, n_steps: int, *, tau, q: Optional[torch.Tensor] = 
     cfg = dict(n_steps=int(n_steps), nx=nx, ny=ny, nz=nz, tau=F.scalar_float(tau), rho_out=float(rho_out),
                collision=collision, smagorinsky_cs=smagorinsky_cs, solid=solid, guard=guard,
                guard_velocity=guard_velocity, storage=storage, offset=offset, checkpoint=checkpoint,
-               adjoint_storage=adjoint_storage)
+               adjoint_storage=adjoint_storage, layout=layout)
     if int(n_steps) < 1:
         raise ValueError("n_steps must be >= 1")
     return _RolloutFn.apply(f0, q, porosity, u, cfg)

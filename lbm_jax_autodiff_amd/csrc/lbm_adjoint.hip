@@ -23,8 +23,9 @@ template <typename R>
 struct AdjArgs {
   // forward configuration (same meaning as in the forward kernel)
   int nx, ny, nz;
-  uint32_t N;  // cells
-  uint32_t P;  // elements between consecutive populations (state and cotangent buffers)
+  uint32_t N;   // cells
+  uint32_t P;   // elements between consecutive populations (state and cotangent buffers)
+  uint32_t RS;  // elements between consecutive (y, z) rows (state and cotangent buffers)
   FastDiv dnx, dny;
   int smag, bounce, guard, n_links, nlb, ndb;
   R tau, omega, cs;
@@ -70,23 +71,31 @@ __device__ __forceinline__ void adj_coords(const AdjArgs<R>& a, uint32_t c, int&
   z = int(zz);
 }
 
-// cell x - e_I with all three axes wrapped: the entry whose normal consumer is x
+// Element offset of cell (x, y, z) within a population (rows RS apart).
+template <typename R>
+__device__ __forceinline__ uint32_t adj_addr(const AdjArgs<R>& a, int x, int y, int z) {
+  return (uint32_t(z) * uint32_t(a.ny) + uint32_t(y)) * a.RS + uint32_t(x);
+}
+
+// offset of cell x - e_I with all three axes wrapped: the entry whose normal
+// consumer is x
 template <int I, typename R>
 __device__ __forceinline__ uint32_t owner_target(const AdjArgs<R>& a, int x, int y, int z) {
   int sx = x - EX[I], sy = y - EY[I], sz = z - EZ[I];
   sx = sx < 0 ? sx + a.nx : (sx >= a.nx ? sx - a.nx : sx);
   sy = sy < 0 ? sy + a.ny : (sy >= a.ny ? sy - a.ny : sy);
   sz = sz < 0 ? sz + a.nz : (sz >= a.nz ? sz - a.nz : sz);
-  return (uint32_t(sz) * uint32_t(a.ny) + uint32_t(sy)) * uint32_t(a.nx) + uint32_t(sx);
+  return adj_addr(a, sx, sy, sz);
 }
 
+// offset of cell x + e_I, all axes wrapped
 template <int I, typename R>
 __device__ __forceinline__ uint32_t next_cell(const AdjArgs<R>& a, int x, int y, int z) {
   int fx = x + EX[I], fy = y + EY[I], fz = z + EZ[I];
   fx = fx < 0 ? fx + a.nx : (fx >= a.nx ? fx - a.nx : fx);
   fy = fy < 0 ? fy + a.ny : (fy >= a.ny ? fy - a.ny : fy);
   fz = fz < 0 ? fz + a.nz : (fz >= a.nz ? fz - a.nz : fz);
-  return (uint32_t(fz) * uint32_t(a.ny) + uint32_t(fy)) * uint32_t(a.nx) + uint32_t(fx);
+  return adj_addr(a, fx, fy, fz);
 }
 
 template <int I>
@@ -96,17 +105,17 @@ __device__ __forceinline__ bool x_fallback(int x, int nx) {
   else return false;
 }
 
-// Pulled values of a non-link cell x (streaming.py:113-142): x - e_i with all
-// axes wrapped, the cell itself at the x faces.
+// Pulled values of a non-link cell x at offset o (streaming.py:113-142):
+// x - e_i with all axes wrapped, the cell itself at the x faces.
 template <class S>
 __device__ __forceinline__ void pull_dense(const AdjArgs<typename S::R>& a, const typename S::T* __restrict__ P,
-                                           uint32_t c, int x, int y, int z, typename S::R* s) {
+                                           uint32_t o, int x, int y, int z, typename S::R* s) {
   const size_t N = a.P;
-  s[0] = S::ld_c(P, c, a.off[0]);
+  s[0] = S::ld_c(P, o, a.off[0]);
   sfor<Q>([&](auto I) {
     constexpr int i = I;
     if constexpr (i != 0) {
-      const size_t k = i * N + (x_fallback<i>(x, a.nx) ? c : owner_target<i>(a, x, y, z));
+      const size_t k = i * N + (x_fallback<i>(x, a.nx) ? o : owner_target<i>(a, x, y, z));
       s[i] = S::ld_c(P, k, a.off[i]);
     }
   });
@@ -121,11 +130,11 @@ __device__ __forceinline__ void pull_dense(const AdjArgs<typename S::R>& a, cons
 // round trips and keeps no per-direction state in registers afterwards.
 template <class S, int CH>
 __device__ __forceinline__ void pull_link(const AdjArgs<typename S::R>& a, const typename S::T* __restrict__ P,
-                                          uint32_t c, int x, int y, int z, const typename S::R* qv,
+                                          uint32_t o, int x, int y, int z, const typename S::R* qv,
                                           typename S::R* s, typename S::R* dsdq) {
   using R = typename S::R;
   const size_t N = a.P;
-  s[0] = S::ld_c(P, c, a.off[0]);
+  s[0] = S::ld_c(P, o, a.off[0]);
   sfor<(Q - 1 + CH - 1) / CH>([&](auto K) {
     constexpr int k0 = 1 + int(K) * CH;
     R va[CH], vb[CH];
@@ -136,10 +145,10 @@ __device__ __forceinline__ void pull_link(const AdjArgs<typename S::R>& a, const
         const R qo = qv[io];
         const bool low = (qo >= R(0)) & (qo < R(0.5));
         const bool high = qo >= R(0.5);
-        const size_t normal = i * N + (x_fallback<i>(x, a.nx) ? c : owner_target<i>(a, x, y, z));
-        const size_t ka = low ? io * N + next_cell<i>(a, x, y, z) : (high ? i * N + c : normal);
+        const size_t normal = i * N + (x_fallback<i>(x, a.nx) ? o : owner_target<i>(a, x, y, z));
+        const size_t ka = low ? io * N + next_cell<i>(a, x, y, z) : (high ? i * N + o : normal);
         va[J] = S::ld_c(P, ka, low ? a.off[io] : a.off[i]);
-        vb[J] = S::ld_c(P, io * N + c, a.off[io]);
+        vb[J] = S::ld_c(P, io * N + o, a.off[io]);
       }
     });
     sfor<CH>([&](auto J) {
@@ -228,16 +237,17 @@ __device__ __forceinline__ void adjoint_cell(const AdjArgs<typename S::R>& a, co
   const bool skip = !LINK && (fl & 2u);
   int x, y, z;
   adj_coords(a, c, x, y, z);
+  const uint32_t o = adj_addr(a, x, y, z);
   R s[Q];
   uint32_t lmask = 0;  // bit j: direction j is a wall link (q_j >= 0)
   if constexpr (LINK) {
-    pull_link<S, 6>(a, P, c, x, y, z, qv, s, dsdq);
+    pull_link<S, 6>(a, P, o, x, y, z, qv, s, dsdq);
     sfor<Q>([&](auto J) {
       constexpr int j = J;
       if constexpr (j != 0) lmask |= (qv[j] >= R(0)) ? (1u << j) : 0u;
     });
   } else {
-    pull_dense<S>(a, P, c, x, y, z, s);
+    pull_dense<S>(a, P, o, x, y, z, s);
   }
   // the inlet's density depends on the pulled values it keeps
   R rho_in = R(0);
@@ -248,7 +258,7 @@ __device__ __forceinline__ void adjoint_cell(const AdjArgs<typename S::R>& a, co
   R g[Q];
   sfor<Q>([&](auto I) {
     constexpr int i = I;
-    g[i] = LM::ld(lam_in, i * N + c, a.lam_inv);
+    g[i] = LM::ld(lam_in, i * N + o, a.lam_inv);
   });
   if constexpr (MODE == LBM_MODE_FULL) collide_transpose<R, COLL>(a, s, g, c, fl, skip);
   // momentum-exchange force uses S_{t+1}(x)_opp(j) for links j (forces.py:95-118)
@@ -289,7 +299,7 @@ __device__ __forceinline__ void adjoint_cell(const AdjArgs<typename S::R>& a, co
   }
   // owner writes and Bouzidi bookkeeping
   if (skip) return;
-  LM::st(lam_out, c, g[0], a.lam_scale);
+  LM::st(lam_out, o, g[0], a.lam_scale);
   sfor<Q>([&](auto I) {
     constexpr int i = I;
     if constexpr (i != 0) {
@@ -359,16 +369,19 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjoint_collide(AdjArgs<typename 
   const uint32_t c = blockIdx.x * kAdjBlock + threadIdx.x;
   if (c >= a.N) return;
   const size_t N = a.P;
+  int x, y, z;
+  adj_coords(a, c, x, y, z);
+  const uint32_t o = adj_addr(a, x, y, z);
   R s[Q], g[Q];
   sfor<Q>([&](auto I) {
     constexpr int i = I;
-    s[i] = S::ld_c(S0, i * N + c, a.off[i]);
-    g[i] = LM::ld(lam_in, i * N + c, a.lam_inv);
+    s[i] = S::ld_c(S0, i * N + o, a.off[i]);
+    g[i] = LM::ld(lam_in, i * N + o, a.lam_inv);
   });
   collide_transpose<R, COLL>(a, s, g, c, a.flags ? uint32_t(a.flags[c]) : 0u, false);
   sfor<Q>([&](auto I) {
     constexpr int i = I;
-    g_out[i * N + c] = g[i];
+    g_out[i * N + o] = g[i];
   });
 }
 
@@ -404,10 +417,11 @@ template <typename R>
 int make_adj_args(const lbm_step_desc* d, const lbm_adjoint_desc* ad, int mode, AdjArgs<R>& a) {
   if (!d || !ad || d->nx < 2 || d->ny <= 0 || d->nz <= 0) return LBM_E_ARG;
   const long long n = 1LL * d->nx * d->ny * d->nz;
-  const long long pstride = d->pop_stride ? d->pop_stride : n;
-  if (pstride < n || pstride * Q >= (1LL << 31)) return LBM_E_ARG;
+  long long pstride = 0, rstride = 0;
+  if (!state_layout(d, pstride, rstride)) return LBM_E_ARG;
   if (!d->periodic_yz) return LBM_E_ARG;  // the owner-write scheme needs periodic y/z
   a.P = uint32_t(pstride);
+  a.RS = uint32_t(rstride);
   a.nx = d->nx;
   a.ny = d->ny;
   a.nz = d->nz;

@@ -252,6 +252,22 @@ __device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv& f) {
   return (__umulhi(n, f.m) + n) >> f.s;
 }
 
+// Strides of a fused-path state buffer (lbm_step_desc.pop_stride/row_stride,
+// 0 = dense): population i of cell (x, y, z) is element
+// i * pstride + (z * ny + y) * rstride + x.  Accepted: whole populations
+// apart (rstride = nx, pstride >= N) or the populations of a row interleaved
+// (pstride >= nx, rstride >= 19 pstride); every element offset below 2^31.
+inline bool state_layout(const lbm_step_desc* d, long long& pstride, long long& rstride) {
+  const long long nx = d->nx, rows = 1LL * d->ny * d->nz, n = nx * rows;
+  pstride = d->pop_stride ? d->pop_stride : n;
+  rstride = d->row_stride ? d->row_stride : nx;
+  const bool apart = rstride == nx && pstride >= n;
+  const bool rowwise = pstride >= nx && rstride >= Q * pstride;
+  if (!apart && !rowwise) return false;
+  const long long extent = (Q - 1) * pstride + (rows - 1) * rstride + nx;
+  return extent < (1LL << 31);
+}
+
 // Map a hardware workgroup id to a logical tile so that consecutive logical
 // tiles land on the same XCD (workgroups are dealt round-robin to the 8 XCDs
 // of an MI355X).  Keeps the halo cache lines of neighbouring tiles in one L2.

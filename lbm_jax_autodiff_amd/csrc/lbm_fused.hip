@@ -25,8 +25,9 @@ constexpr int kStepBlock = 256;
 template <typename R>
 struct StepArgs {
   int nx, ny, nz;
-  uint32_t N;  // cells
-  uint32_t P;  // elements between consecutive populations of a state buffer
+  uint32_t N;   // cells
+  uint32_t P;   // elements between consecutive populations of a state buffer
+  uint32_t RS;  // elements between consecutive (y, z) rows of a state buffer
   FastDiv dnx, dny;
   int coll, smag, bounce, guard, periodic, n_links, nlb, ndb;
   R tau, omega, cs;
@@ -49,16 +50,29 @@ __device__ __forceinline__ void cell_coords(const StepArgs<R>& a, uint32_t c, in
   z = int(zz);
 }
 
+// Element offset of cell (x, y, z) within a population: rows are RS
+// elements apart (RS = nx when the populations are whole arrays apart, 19 nx
+// or more when the populations of a row are interleaved).
+template <typename R>
+__device__ __forceinline__ uint32_t cell_addr(const StepArgs<R>& a, int x, int y, int z) {
+  return (uint32_t(z) * uint32_t(a.ny) + uint32_t(y)) * a.RS + uint32_t(x);
+}
+template <typename R>
+__device__ __forceinline__ uint32_t cell_addr(const StepArgs<R>& a, uint32_t c) {
+  if (a.RS == uint32_t(a.nx)) return c;
+  return c + fdiv(c, a.dnx) * (a.RS - uint32_t(a.nx));
+}
+
 // Neighbour rows of a cell, wrapped (periodic) or flagged (clamped).
 struct Rows {
-  uint32_t ym, yp, zm, zp;  // row offsets (in cells) of y-1, y+1, z-1, z+1
+  uint32_t ym, yp, zm, zp;  // element offsets of the rows y-1, y+1, z-1, z+1
   bool ym_ok, yp_ok, zm_ok, zp_ok;
 };
 
 template <typename R>
 __device__ __forceinline__ Rows make_rows(const StepArgs<R>& a, int y, int z) {
   Rows r;
-  const uint32_t nx = uint32_t(a.nx), plane = uint32_t(a.nx) * uint32_t(a.ny);
+  const uint32_t nx = a.RS, plane = a.RS * uint32_t(a.ny);
   r.ym = (y == 0) ? (uint32_t(a.ny - 1) * nx) : uint32_t(0) - nx;
   r.yp = (y == a.ny - 1) ? uint32_t(0) - uint32_t(a.ny - 1) * nx : nx;
   r.zm = (z == 0) ? (uint32_t(a.nz - 1) * plane) : uint32_t(0) - plane;
@@ -70,10 +84,11 @@ __device__ __forceinline__ Rows make_rows(const StepArgs<R>& a, int y, int z) {
   return r;
 }
 
-// Source cell of the normal pull for direction I (streaming.py:113-142):
-// x - e_I with y/z wrapped; the cell itself where the reference keeps the
-// cell's own value (x faces, and y/z faces when not periodic).  Offsets are
-// added modulo 2^32, so a wrapped row is a plain unsigned addition.
+// Element offset of the source of the normal pull for direction I
+// (streaming.py:113-142): x - e_I with y/z wrapped; the cell itself (c, its
+// own offset) where the reference keeps the cell's own value (x faces, and
+// y/z faces when not periodic).  Offsets are added modulo 2^32, so a wrapped
+// row is a plain unsigned addition.
 template <int I, typename R>
 __device__ __forceinline__ uint32_t pull_src(const StepArgs<R>& a, const Rows& r, uint32_t c, int x) {
   constexpr int ex = EX[I], ey = EY[I], ez = EZ[I];
@@ -100,14 +115,15 @@ __device__ __forceinline__ uint32_t pull_src(const StepArgs<R>& a, const Rows& r
   return s;
 }
 
-// Cell x + e_I wrapped periodically in all three axes (streaming.py:166-172).
+// Offset of cell x + e_I wrapped periodically in all three axes
+// (streaming.py:166-172).
 template <int I, typename R>
 __device__ __forceinline__ uint32_t wrap_next(const StepArgs<R>& a, int x, int y, int z) {
   int fx = x + EX[I], fy = y + EY[I], fz = z + EZ[I];
   fx = fx < 0 ? fx + a.nx : (fx >= a.nx ? fx - a.nx : fx);
   fy = fy < 0 ? fy + a.ny : (fy >= a.ny ? fy - a.ny : fy);
   fz = fz < 0 ? fz + a.nz : (fz >= a.nz ? fz - a.nz : fz);
-  return (uint32_t(fz) * uint32_t(a.ny) + uint32_t(fy)) * uint32_t(a.nx) + uint32_t(fx);
+  return cell_addr(a, fx, fy, fz);
 }
 
 // Zou-He planes and the post-collision guard on a pulled S (state.py:152-169).
@@ -132,13 +148,14 @@ __device__ __forceinline__ void collide_cell(const StepArgs<R>& a, R* s, uint32_
   }
 }
 
+// s -> the 19 populations at cell offset o
 template <class S>
 __device__ __forceinline__ void store_cell(const StepArgs<typename S::R>& a, typename S::T* __restrict__ dst,
-                                           uint32_t c, const typename S::R* s) {
+                                           uint32_t o, const typename S::R* s) {
   const size_t N = a.P;
   sfor<Q>([&](auto I) {
     constexpr int i = I;
-    S::st(dst, i * N + c, s[i], a.off[i]);
+    S::st(dst, i * N + o, s[i], a.off[i]);
   });
 }
 
@@ -156,6 +173,7 @@ __device__ __forceinline__ void dense_vec(const StepArgs<typename S::R>& a, cons
   if (c0 >= a.N) return;
   int x0, y, z;
   cell_coords(a, c0, x0, y, z);
+  const uint32_t o0 = cell_addr(a, x0, y, z);  // offset of cell c0 (its V cells are adjacent)
   const int lane = threadIdx.x & 63;
   const bool row_first = (x0 == 0), row_last = (x0 + V == a.nx);
   R s[V][Q];
@@ -163,7 +181,7 @@ __device__ __forceinline__ void dense_vec(const StepArgs<typename S::R>& a, cons
     sfor<Q>([&](auto I) {
       constexpr int i = I;
       R blk[V];
-      S::template ldv<V>(src, i * N + c0, a.off[i], blk);
+      S::template ldv<V>(src, i * N + o0, a.off[i], blk);
 #pragma unroll
       for (int j = 0; j < V; ++j) s[j][i] = blk[j];
     });
@@ -174,13 +192,13 @@ __device__ __forceinline__ void dense_vec(const StepArgs<typename S::R>& a, cons
       constexpr int ex = EX[i], ey = EY[i], ez = EZ[i];
       // aligned block of the source row (x not shifted), or the own row when
       // a clamped y/z face keeps the cell's own values
-      uint32_t base = c0;
+      uint32_t base = o0;
       bool row_ok = true;
       if constexpr (ey == 1) { row_ok = row_ok && rows.ym_ok; base += rows.ym; }
       else if constexpr (ey == -1) { row_ok = row_ok && rows.yp_ok; base += rows.yp; }
       if constexpr (ez == 1) { row_ok = row_ok && rows.zm_ok; base += rows.zm; }
       else if constexpr (ez == -1) { row_ok = row_ok && rows.zp_ok; base += rows.zp; }
-      if (!row_ok) base = c0;
+      if (!row_ok) base = o0;
       R blk[V];
       S::template ldv<V>(src, i * N + base, a.off[i], blk);
       if constexpr (ex == 0) {
@@ -190,7 +208,7 @@ __device__ __forceinline__ void dense_vec(const StepArgs<typename S::R>& a, cons
         // cell x0 + j pulls x0 + j - 1
         R prev = __shfl_up(blk[V - 1], 1, 64);
         if (row_ok && !row_first && lane == 0) prev = S::ld_c(src, i * N + base - 1, a.off[i]);
-        if (row_first && row_ok) prev = S::ld_c(src, i * N + c0, a.off[i]);  // x = 0 keeps its own value
+        if (row_first && row_ok) prev = S::ld_c(src, i * N + o0, a.off[i]);  // x = 0 keeps its own value
         s[0][i] = row_ok ? prev : blk[0];
 #pragma unroll
         for (int j = 1; j < V; ++j) s[j][i] = row_ok ? blk[j - 1] : blk[j];
@@ -198,7 +216,7 @@ __device__ __forceinline__ void dense_vec(const StepArgs<typename S::R>& a, cons
         // cell x0 + j pulls x0 + j + 1
         R next = __shfl_down(blk[0], 1, 64);
         if (row_ok && !row_last && lane == 63) next = S::ld_c(src, i * N + base + V, a.off[i]);
-        if (row_last && row_ok) next = S::ld_c(src, i * N + c0 + V - 1, a.off[i]);  // x = nx-1 keeps its own
+        if (row_last && row_ok) next = S::ld_c(src, i * N + o0 + V - 1, a.off[i]);  // x = nx-1 keeps its own
 #pragma unroll
         for (int j = 0; j < V - 1; ++j) s[j][i] = row_ok ? blk[j + 1] : blk[j];
         s[V - 1][i] = row_ok ? next : blk[V - 1];
@@ -240,11 +258,11 @@ __device__ __forceinline__ void dense_vec(const StepArgs<typename S::R>& a, cons
 #pragma unroll
     for (int j = 0; j < V; ++j) o[j] = s[j][i];
     if (!has_link) {
-      S::template stv<V>(dst, i * N + c0, o, a.off[i]);
+      S::template stv<V>(dst, i * N + o0, o, a.off[i]);
     } else {
 #pragma unroll
       for (int j = 0; j < V; ++j)
-        if (!((fl >> (8 * j)) & 2u)) S::st(dst, i * N + c0 + j, o[j], a.off[i]);
+        if (!((fl >> (8 * j)) & 2u)) S::st(dst, i * N + o0 + j, o[j], a.off[i]);
     }
   });
 }
@@ -288,9 +306,10 @@ __device__ __forceinline__ void link_cell_update(const StepArgs<typename S::R>& 
   for (int i = 1; i < Q; ++i) asm volatile("" : "+v"(qv[i]));
   int x, y, z;
   cell_coords(a, c, x, y, z);
+  const uint32_t o = cell_addr(a, x, y, z);
   const Rows rows = make_rows(a, y, z);
   R s[Q];
-  s[0] = S::ld_c(src, c, a.off[0]);
+  s[0] = S::ld_c(src, o, a.off[0]);
   sfor<(Q - 1 + CH - 1) / CH>([&](auto K) {
     constexpr int k0 = 1 + int(K) * CH;
     R va[CH], vb[CH];
@@ -302,9 +321,9 @@ __device__ __forceinline__ void link_cell_update(const StepArgs<typename S::R>& 
         const bool low = (qo >= R(0)) & (qo < R(0.5));
         const bool high = qo >= R(0.5);
         const size_t ka = low ? io * N + wrap_next<i>(a, x, y, z)
-                              : (high ? i * N + c : i * N + pull_src<i>(a, rows, c, x));
+                              : (high ? i * N + o : i * N + pull_src<i>(a, rows, o, x));
         va[J] = S::ld_c(src, ka, low ? a.off[io] : a.off[i]);
-        vb[J] = S::ld_c(src, io * N + c, a.off[io]);
+        vb[J] = S::ld_c(src, io * N + o, a.off[io]);
       }
     });
     sfor<CH>([&](auto J) {
@@ -351,7 +370,7 @@ __device__ __forceinline__ void link_cell_update(const StepArgs<typename S::R>& 
   f_out[1] = f[1];
   f_out[2] = f[2];
   if (MODE == LBM_MODE_FULL) collide_cell<R, COLL>(a, s, c, a.flags[c]);
-  store_cell<S>(a, dst, c, s);
+  store_cell<S>(a, dst, o, s);
 }
 
 template <class S, int COLL, int MODE, int V>
@@ -404,25 +423,28 @@ __global__ __launch_bounds__(kStepBlock) __attribute__((amdgpu_waves_per_eu(step
   // front of them.  Link cells are computed and then not stored.
   const uint32_t fl = a.flags ? uint32_t(a.flags[c]) : 0u;
   R s[Q];
+  uint32_t o;
   if (MODE == LBM_MODE_COLLIDE) {
+    o = cell_addr(a, c);
     sfor<Q>([&](auto I) {
       constexpr int i = I;
-      s[i] = S::ld(src, i * N + c, a.off[i]);
+      s[i] = S::ld(src, i * N + o, a.off[i]);
     });
     collide_cell<R, COLL>(a, s, c, fl);
   } else {
     int x, y, z;
     cell_coords(a, c, x, y, z);
+    o = cell_addr(a, x, y, z);
     const Rows rows = make_rows(a, y, z);
     sfor<Q>([&](auto I) {
       constexpr int i = I;
-      s[i] = S::ld_c(src, i * N + pull_src<i>(a, rows, c, x), a.off[i]);
+      s[i] = S::ld_c(src, i * N + pull_src<i>(a, rows, o, x), a.off[i]);
     });
     boundaries(a, s, x);
     if (MODE == LBM_MODE_FULL) collide_cell<R, COLL>(a, s, c, fl);
     if (fl & 2u) return;  // owned by a link workgroup
   }
-  store_cell<S>(a, dst, c, s);
+  store_cell<S>(a, dst, o, s);
 }
 
 __global__ void k_force_reduce(const double* __restrict__ part, int T, int slots, double* __restrict__ force) {
@@ -446,7 +468,7 @@ __global__ void k_convert(const typename SI::T* __restrict__ in, typename SO::T*
   const size_t k = blockIdx.x * size_t(blockDim.x) + threadIdx.x;
   if (k >= total) return;
   const int i = int(k / a.N);
-  const size_t e = size_t(i) * a.P + (k - size_t(i) * a.N);
+  const size_t e = size_t(i) * a.P + cell_addr(a, uint32_t(k - size_t(i) * a.N));
   using RI = typename SI::R;
   using RO = typename SO::R;
   const RI v = SI::ld_c(in, e, RI(a.off[i]));
@@ -457,9 +479,10 @@ template <typename R>
 int make_args(const lbm_step_desc* d, int mode, StepArgs<R>& a) {
   if (!d || d->nx <= 0 || d->ny <= 0 || d->nz <= 0) return LBM_E_ARG;
   const long long n = 1LL * d->nx * d->ny * d->nz;
-  const long long pstride = d->pop_stride ? d->pop_stride : n;
-  if (pstride < n || pstride * Q >= (1LL << 31)) return LBM_E_ARG;
+  long long pstride = 0, rstride = 0;
+  if (!state_layout(d, pstride, rstride)) return LBM_E_ARG;
   a.P = uint32_t(pstride);
+  a.RS = uint32_t(rstride);
   a.nx = d->nx;
   a.ny = d->ny;
   a.nz = d->nz;
@@ -498,9 +521,10 @@ int make_args(const lbm_step_desc* d, int mode, StepArgs<R>& a) {
 // Cells per thread of the dense sweep: 2 or 4 when requested and nx allows,
 // else 1 (the fastest on gfx950: the sweep is arithmetic-bound and wider
 // threads only raise register pressure).
-inline int choose_vec(const lbm_step_desc* d, uint32_t pstride) {
+inline int choose_vec(const lbm_step_desc* d, uint32_t pstride, uint32_t rstride) {
   const int want = d->cells_per_thread;
-  if (want == 2 || want == 4) return (d->nx % want == 0 && pstride % uint32_t(want) == 0) ? want : 1;
+  if (want == 2 || want == 4)
+    return (d->nx % want == 0 && pstride % uint32_t(want) == 0 && rstride % uint32_t(want) == 0) ? want : 1;
   return 1;
 }
 
@@ -521,7 +545,7 @@ int launch_step(const lbm_step_desc* d, int mode, const void* src, void* dst, do
   const int rc = make_args(d, mode, a);
   if (rc) return rc;
   if (!src || !dst || src == dst) return LBM_E_ARG;
-  const int V = choose_vec(d, a.P);
+  const int V = choose_vec(d, a.P, a.RS);
   const long long cells_per_block = 1LL * kStepBlock * V;
   a.ndb = int((1LL * a.N + cells_per_block - 1) / cells_per_block);
   const auto* in = static_cast<const typename S::T*>(src);

@@ -103,15 +103,19 @@ class Lattice:
     (f_eq(1, inlet_velocity)) or 19 numbers.  ``cells_per_thread`` (1, 2
     or 4 when it divides nx; 0 = 1, the fastest) sets how many x-consecutive
     cells one thread of the dense sweep updates with vector loads.
-    ``pop_stride`` is the element distance between consecutive populations
-    of the state buffers (default: the library's padded recommendation;
-    N gives the dense (19, nz, ny, nx) layout).
+    ``layout`` places the state buffers of the fused path: "rows" (default)
+    interleaves the populations of every x-row, [nz][ny][19][nx], so a row's
+    19 populations are one contiguous block; "apart" keeps whole populations
+    apart, [19][pop_stride] with cells x-fastest.  ``pop_stride`` is the
+    element distance between consecutive populations (rows: nx by default;
+    apart: the library's padded recommendation by default, N for the dense
+    (19, nz, ny, nx) array).
     """
 
     def __init__(self, nx, ny, nz, *, tau, inlet_velocity=(0.05, 0.0, 0.0), rho_out=1.0, collision="bgk",
                  smagorinsky_cs=None, solid=None, porosity=None, q=None, guard="none", guard_velocity=None,
                  periodic_yz=True, storage="f32", offset="weights", device=None, dqdr=None,
-                 cells_per_thread=0, pop_stride=None):
+                 cells_per_thread=0, pop_stride=None, layout="rows"):
         if storage not in _STORAGE:
             raise ValueError(f"storage must be one of {sorted(_STORAGE)}")
         if guard not in _GUARD:
@@ -184,19 +188,31 @@ class Lattice:
         d.link_q = L.ptr(self.links.q)
         d.cells_per_thread = int(cells_per_thread)
         lib = L.load()
-        self.P = int(lib.lbm_state_stride(ctypes.byref(d))) if pop_stride is None else int(pop_stride)
-        if self.P < self.N or 19 * self.P >= 2 ** 31:
-            raise ValueError(f"pop_stride must be in [N, 2^31 / 19); got {self.P} for N = {self.N}")
+        if layout == "rows":
+            self.P = self.nx if pop_stride is None else int(pop_stride)
+            self.RS = 19 * self.P
+            ok = self.P >= self.nx
+        elif layout == "apart":
+            self.P = int(lib.lbm_state_stride(ctypes.byref(d))) if pop_stride is None else int(pop_stride)
+            self.RS = self.nx
+            ok = self.P >= self.N
+        else:
+            raise ValueError("layout must be 'rows' or 'apart'")
+        self.layout = layout
+        self.extent = 18 * self.P + (self.ny * self.nz - 1) * self.RS + self.nx
+        if not ok or self.extent >= 2 ** 31:
+            raise ValueError(f"pop_stride {self.P} does not fit layout {layout!r} for grid {nx}x{ny}x{nz}")
         d.pop_stride = self.P
+        d.row_stride = self.RS
         self.desc = d
         self.slots = int(lib.lbm_step_force_slots(ctypes.byref(d)))
 
     # ---- state buffers ---------------------------------------------------
     def new_state(self, dtype: Optional[torch.dtype] = None) -> torch.Tensor:
-        """Uninitialised (19, nz, ny, nx) buffer with population stride P
+        """Uninitialised buffer in the state layout, viewed as (19, nz, ny, nx)
         (storage dtype by default; cotangent buffers pass their real dtype)."""
-        flat = torch.empty(19 * self.P, dtype=dtype or self.buf_dtype, device=self.device)
-        return flat.as_strided((19, self.nz, self.ny, self.nx), (self.P, self.ny * self.nx, self.nx, 1))
+        flat = torch.empty(self.extent, dtype=dtype or self.buf_dtype, device=self.device)
+        return flat.as_strided((19, self.nz, self.ny, self.nx), (self.P, self.ny * self.RS, self.RS, 1))
 
     def clone_state(self, buf: torch.Tensor) -> torch.Tensor:
         out = self.new_state(buf.dtype)

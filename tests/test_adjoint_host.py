@@ -159,7 +159,7 @@ def test_stream_transpose_decomposition():
     has = (qs[1:] >= 0).any(0)
     cells = torch.nonzero(has).reshape(-1)
     links = types.SimpleNamespace(n=int(cells.numel()), cell=cells.to(torch.int32), q=qs[1:, cells].clone())
-    lat = types.SimpleNamespace(links=links, nx=nx, ny=ny, nz=nz, N=N, device=torch.device("cpu"))
+    lat = types.SimpleNamespace(links=links, nx=nx, ny=ny, nz=nz, N=N, P=N, RS=nx, device=torch.device("cpu"))
     tgt, start, src, coef = A.scatter_plan(lat)
     mu_s = mu.permute(0, 3, 2, 1).contiguous().reshape(19, N)
     mu_link = torch.zeros((19, links.n), dtype=torch.float64)
@@ -186,7 +186,7 @@ def test_scatter_plan_is_sorted_and_complete():
     qs = q.permute(0, 3, 2, 1).contiguous().reshape(19, N)
     cells = torch.nonzero((qs[1:] >= 0).any(0)).reshape(-1)
     links = types.SimpleNamespace(n=int(cells.numel()), cell=cells.to(torch.int32), q=qs[1:, cells].clone())
-    lat = types.SimpleNamespace(links=links, nx=nx, ny=ny, nz=nz, N=N, device=torch.device("cpu"))
+    lat = types.SimpleNamespace(links=links, nx=nx, ny=ny, nz=nz, N=N, P=N, RS=nx, device=torch.device("cpu"))
     tgt, start, src, coef = A.scatter_plan(lat)
     t = tgt.numpy()
     assert np.all(np.diff(t) > 0)
@@ -198,18 +198,40 @@ def test_scatter_plan_is_sorted_and_complete():
     assert n_hi <= n_links
 
 
-def test_scatter_plan_uses_population_stride():
-    """Targets address population i of cell c at i * P + c for a padded stride P."""
+def _plan_by_entry(plan, decode):
+    """{(population, cell): sorted [(src, coef)]} of a scatter plan."""
+    tgt, start, src, coef = plan
+    out = {}
+    for k in range(tgt.numel()):
+        ent = [(int(src[e]), float(coef[e])) for e in range(int(start[k]), int(start[k + 1]))]
+        out[decode(int(tgt[k]))] = sorted(ent)
+    return out
+
+
+def test_scatter_plan_follows_the_state_layout():
+    """Targets address population i of cell (x, y, z) at
+    i * P + (z * ny + y) * RS + x: the dense array, a padded population stride
+    and the row-interleaved layout give the same entries."""
     nx, ny, nz = 6, 5, 4
     q = random_q(nx, ny, nz, seed=5, p_wall=0.3)
     N = nx * ny * nz
     qs = q.permute(0, 3, 2, 1).contiguous().reshape(19, N)
     cells = torch.nonzero((qs[1:] >= 0).any(0)).reshape(-1)
     links = types.SimpleNamespace(n=int(cells.numel()), cell=cells.to(torch.int32), q=qs[1:, cells].clone())
-    dense = types.SimpleNamespace(links=links, nx=nx, ny=ny, nz=nz, N=N, device=torch.device("cpu"))
-    P = N + 37
-    padded = types.SimpleNamespace(links=links, nx=nx, ny=ny, nz=nz, N=N, P=P, device=torch.device("cpu"))
-    t0, s0, src0, c0 = A.scatter_plan(dense)
-    t1, s1, src1, c1 = A.scatter_plan(padded)
-    assert torch.equal(t1 // P, t0 // N) and torch.equal(t1 % P, t0 % N)
-    assert torch.equal(s0, s1) and torch.equal(src0, src1) and torch.equal(c0, c1)
+
+    def lattice(P, RS):
+        return types.SimpleNamespace(links=links, nx=nx, ny=ny, nz=nz, N=N, P=P, RS=RS, device=torch.device("cpu"))
+
+    def decoder(P, RS):
+        def dec(t):
+            if RS == nx:
+                return t // P, t % P
+            row, rem = divmod(t, RS)
+            return rem // P, row * nx + rem % P
+        return dec
+
+    ref = _plan_by_entry(A.scatter_plan(lattice(N, nx)), decoder(N, nx))
+    for P, RS in ((N + 37, nx), (nx, 19 * nx), (nx + 2, 19 * (nx + 2) + 5)):
+        plan = A.scatter_plan(lattice(P, RS))
+        assert np.all(np.diff(plan[0].numpy()) > 0)  # sorted, unique targets
+        assert _plan_by_entry(plan, decoder(P, RS)) == ref

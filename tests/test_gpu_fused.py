@@ -147,9 +147,10 @@ def test_poiseuille_kat(dev):
 
 @pytest.mark.parametrize("vec", [1, 4])
 @pytest.mark.parametrize("storage", ["f32", "f16", "f64"])
-def test_population_stride_does_not_change_results(dev, storage, vec):
-    """The padded population stride (default) and the dense layout (pop_stride
-    = N) give bit-identical states and forces."""
+def test_state_layout_does_not_change_results(dev, storage, vec):
+    """The row-interleaved layout (default, also with padded rows), the padded
+    population stride and the dense (19, nz, ny, nx) array give bit-identical
+    states and forces."""
     steps = 5
     f0 = near_eq_state(NX, NY, NZ, amp=2e-3)
     q = random_q(NX, NY, NZ, seed=4, p_wall=0.1)
@@ -157,9 +158,10 @@ def test_population_stride_does_not_change_results(dev, storage, vec):
     solid = (torch.rand((NX, NY, NZ), generator=g) < 0.1).to(torch.int32)
     dt = torch.float64 if storage == "f64" else torch.float32
     outs = []
-    for stride in (None, NX * NY * NZ, NX * NY * NZ + 4 * 37):
+    for layout, stride in (("rows", None), ("rows", NX + 4), ("apart", None), ("apart", NX * NY * NZ),
+                           ("apart", NX * NY * NZ + 4 * 37)):
         lat = Lattice(NX, NY, NZ, tau=0.6, collision="mrt", solid=solid.to(dev), q=q.to(dev), guard="pre",
-                      storage=storage, device=dev, cells_per_thread=vec, pop_stride=stride)
+                      storage=storage, device=dev, cells_per_thread=vec, pop_stride=stride, layout=layout)
         outs.append([cpu(t) for t in lat.run(f0.to(dt).to(dev), steps)])
     assert outs[0][0].shape == (19, NX, NY, NZ)
     for o in outs[1:]:
@@ -167,8 +169,12 @@ def test_population_stride_does_not_change_results(dev, storage, vec):
             assert torch.equal(a, b)
 
 
-def test_default_population_stride_is_padded(dev):
+def test_default_state_layouts(dev):
     lat = Lattice(256, 128, 128, tau=0.6, device=dev)
+    assert lat.layout == "rows" and lat.P == 256 and lat.RS == 19 * 256
+    buf = lat.new_state()
+    assert buf.stride() == (256, 128 * 19 * 256, 19 * 256, 1) and buf.storage().size() == 19 * lat.N
+    lat = Lattice(256, 128, 128, tau=0.6, device=dev, layout="apart")
     assert lat.P > lat.N and lat.P % 8192 == 1024
     buf = lat.new_state()
     assert buf.stride() == (lat.P, 256 * 128, 256, 1)
