@@ -33,7 +33,7 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE / WRITE_SIZE counter passes scaled by a 1 GiB copy
 # (profiles/k_step_mrt_f32_256x128x128_hbm_traffic.md); re-measure when the
 # kernel changes
-MEASURED_TRAFFIC_BYTES = 652.1e6
+MEASURED_TRAFFIC_BYTES = 651.4e6
 NX, NY, NZ = 256, 128, 128
 
 
@@ -130,14 +130,15 @@ def dcd_dr_study(steps, device):
     res = {}
     torch.cuda.empty_cache()
     for st in ("f32", "f16", "f64"):
-        # two identical runs; the second (allocator warm) is reported
-        for _ in range(2):
-            r = sphere_loss_and_grad(NX, NY, NZ, steps, (0.0, 0.0, 0.0), 0.5, storage=st, checkpoint=100,
-                                     avg_last=int(0.3 * steps), guard="pre")
+        # a 200-step run of the same pipeline warms the allocator and the plans
+        for n in (200, steps):
+            r = sphere_loss_and_grad(NX, NY, NZ, n, (0.0, 0.0, 0.0), 0.5, storage=st, checkpoint=100,
+                                     avg_last=int(0.3 * n), guard="pre")
         res[st] = r
+        torch.cuda.empty_cache()
         log(f"dCd/dr study {st}: {r['fwd_adj_s']:.2f} s, Cd={r['loss']:.7f}, dCd/dr={r['dloss_dr_world']:.7f}")
     ref = res["f64"]
-    out = {"steps": steps, "checkpoint": 100, "timing": "second of two identical runs"}
+    out = {"steps": steps, "checkpoint": 100, "timing": "after a 200-step warm-up run of the same pipeline"}
     for st in ("f32", "f16", "f64"):
         r = res[st]
         out[st] = {"s": round(r["fwd_adj_s"], 3), "cd": r["loss"], "dcd_dr": r["dloss_dr_world"],
@@ -198,8 +199,8 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--adj-steps", type=int, default=200)
-    ap.add_argument("--study-steps", type=int, default=2800,
-                    help="steps of the dCd/dr precision study (0 skips it; the full study is 28000)")
+    ap.add_argument("--study-steps", type=int, default=28000,
+                    help="steps of the dCd/dr precision study (0 skips it)")
     ap.add_argument("--no-extras", action="store_true", help="forward float32 only")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (known bytes) for hardware-counter scaling")
