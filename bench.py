@@ -119,7 +119,7 @@ def time_forward_adjoint(storage, steps, device, checkpoint="all"):
     (g,) = torch.autograd.grad(loss, r)
     e1.record()
     torch.cuda.synchronize()
-    return e0.elapsed_time(e1) * 1e-3, float(g), float(loss)
+    return e0.elapsed_time(e1) * 1e-3, float(g), float(loss.detach())
 
 
 def dcd_dr_study(steps, device):

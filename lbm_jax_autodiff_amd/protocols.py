@@ -317,7 +317,7 @@ def main(argv=None):
         res = sphere_loss_and_grad(nx, ny, nz, steps, (0.0, 0.0, 0.0), 0.5, storage=args.storage,
                                    checkpoint=ck, avg_last=int(0.3 * steps), guard="pre",
                                    porosity_mode="soft" if soft else "hard", sharpness=10.0,
-                                   smagorinsky_cs=0.1 if soft else None)
+                                   smagorinsky_cs=0.1 if soft else None, adjoint_storage=args.adjoint_storage)
         res["config"] = "c5-" + args.mode
         _emit(res)
 
