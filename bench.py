@@ -15,6 +15,10 @@ around each launch, half-storage forward MLUPS, forward+adjoint MLUPS
 its wall-time ratio and errors against float64, and the CPU oracle (torch,
 float32) timed on the host over a bounded sample.
 
+Each timed loop follows W untimed warm-up steps and further untimed steps
+until 0.25 s have passed (fresh buffers run up to 10% slower for the first
+tens of milliseconds).
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--skip-cpu]
 """
 
@@ -62,14 +66,24 @@ def initial_state(lat, device):
     return lbm.equilibrium(rho, u)
 
 
+WARM_S = 0.25  # seconds of untimed steps, at least, before a timed loop
+
+
 def time_forward(lat, f0, steps, warmup):
-    """Returns (ms_per_step over the timed loop, avg kernel ms from per-launch events)."""
+    """Returns (ms_per_step over the timed loop, avg kernel ms from per-launch
+    events, wall seconds).  The untimed warm-up is `warmup` steps and then
+    more steps until WARM_S seconds have passed: on fresh buffers the first
+    tens of milliseconds of steps run up to 10% slower."""
     s = lat.to_state(f0)
     a, b = lat.new_state(), lat.new_state()
     lat.collide(s, a)
-    fpart = torch.zeros((max(steps, warmup), max(lat.slots, 1), 3), dtype=torch.float64, device=lat.device)
+    fpart = torch.zeros((max(steps, warmup, 100), max(lat.slots, 1), 3), dtype=torch.float64, device=lat.device)
     cur = lat.pingpong(a, a, b, warmup, fpart[:warmup] if lat.slots else None)
     torch.cuda.synchronize()
+    t_w = time.perf_counter()
+    while time.perf_counter() - t_w < WARM_S:
+        cur = lat.pingpong(cur, a, b, 100, fpart[:100] if lat.slots else None)
+        torch.cuda.synchronize()
     t0 = time.perf_counter()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
