@@ -331,9 +331,81 @@ int lbm_adjoint_chain(const lbm_step_desc* d, const lbm_adjoint_desc* a, int n, 
                       void* lam_a, void* lam_b, const double* gF, lbm_stream_t stream);
 
 /* Convert between storage formats (e.g. float32 <-> half deviations) for a
- * full state; src and dst both use d->pop_stride. */
+ * full state; src and dst both use d->pop_stride and d->row_stride. */
 int lbm_convert_state(const lbm_step_desc* d, int src_dtype, const void* src,
                       int dst_dtype, void* dst, lbm_stream_t stream);
+
+/* ---- Reverse-mode products of the per-operator surface ---------------------
+ * Each replaces what jax.grad does through the cited reference function
+ * (JAX's transposition of its jnp graph; the reference has no hand-written
+ * derivatives).  Dense (19, nz, ny, nx) / (nz, ny, nx) / (3, nz, ny, nx)
+ * arrays as in the forward per-operator entry points; g_out is the
+ * cotangent of the forward output, the g_* outputs are the cotangents of the
+ * inputs (optional ones may be NULL).  Every output entry is written by one
+ * thread: deterministic, no atomics. */
+
+/* equilibrium (jax_lbm/core/equilibrium.py:90-151): g_rho, g_u from g_feq. */
+int lbm_equilibrium_vjp(int dtype, const void* rho, const void* u, const void* g_feq,
+                        void* g_rho, void* g_u, int nx, int ny, int nz, lbm_stream_t stream);
+
+/* compute_density (equilibrium.py:22-48): g_f_i = g_rho. */
+int lbm_density_vjp(int dtype, const void* g_rho, void* g_f, int nx, int ny, int nz,
+                    lbm_stream_t stream);
+
+/* compute_velocity with rho an input (equilibrium.py:51-87): g_f, and g_rho
+ * (optional). */
+int lbm_velocity_vjp(int dtype, const void* f, const void* rho, const void* g_u, void* g_f,
+                     void* g_rho, int nx, int ny, int nz, lbm_stream_t stream);
+
+/* bgk / mrt (collision.py:46-184), collision = LBM_COLL_BGK / LBM_COLL_MRT,
+ * tau scalar or tau_field: g_f, and the per-cell dL/dtau (optional). */
+int lbm_collide_vjp(int dtype, int collision, const void* f, double tau, const void* tau_field,
+                    const void* g_out, void* g_f, void* g_tau, int nx, int ny, int nz,
+                    lbm_stream_t stream);
+
+/* smagorinsky_tau (collision.py:187-270): g_f from the cotangent of tau_eff,
+ * and the per-cell dL/dtau (optional).  At |Pi_neq| = 0 the square root's
+ * derivative is taken as 0 (JAX gives NaN there). */
+int lbm_smagorinsky_tau_vjp(int dtype, const void* f, double tau, double cs, const void* g_tau_eff,
+                            void* g_f, void* g_tau, int nx, int ny, int nz, lbm_stream_t stream);
+
+/* soft_bounce_back (jax_lbm/geometry/smooth.py:118-155): g_f, g_porosity (optional). */
+int lbm_soft_bounce_back_vjp(int dtype, const void* f, const void* porosity, const void* g_out,
+                             void* g_f, void* g_porosity, int nx, int ny, int nz,
+                             lbm_stream_t stream);
+
+/* hard bounce where solid >= 1 (jax_lbm/state.py:146-149). */
+int lbm_hard_bounce_vjp(int dtype, const int32_t* solid, const void* g_out, void* g_f, int nx,
+                        int ny, int nz, lbm_stream_t stream);
+
+/* stream with Bouzidi links (jax_lbm/core/streaming.py:40-185): g_f, and
+ * g_q (optional; needs f) on the taken branch inside the clip bounds. */
+int lbm_stream_vjp(int dtype, const void* f, const void* q, int periodic_yz, const void* g_out,
+                   void* g_f, void* g_q, int nx, int ny, int nz, lbm_stream_t stream);
+
+/* zou_he_inlet (jax_lbm/core/boundary.py:35-124), in place on g (a copy of
+ * the output cotangent, which becomes the input cotangent); per (y, z) row
+ * dL/du (3 doubles, optional) and dL/drho_target (optional). */
+int lbm_zou_he_inlet_vjp(int dtype, const void* f, double ux, double uy, double uz,
+                         int has_rho_target, double rho_target, void* g, double* g_u_rows,
+                         double* g_rho_target_rows, int nx, int ny, int nz, lbm_stream_t stream);
+
+/* zou_he_outlet (boundary.py:127-200), in place on g; per-row dL/drho_out (optional). */
+int lbm_zou_he_outlet_vjp(int dtype, double rho_out, void* g, double* g_rho_out_rows, int nx,
+                          int ny, int nz, lbm_stream_t stream);
+
+/* momentum_exchange(_decomposed) (jax_lbm/core/forces.py:37-196): g_post and
+ * g_streamed from dL/dF (3 doubles) and, for the pressure part, dL/dF_p
+ * (3 doubles, optional; needs f_post). */
+int lbm_momentum_exchange_vjp(int dtype, const void* f_post, const void* q, const double* g_force,
+                              const double* g_pressure, void* g_post, void* g_streamed, int nx,
+                              int ny, int nz, lbm_stream_t stream);
+
+/* drag_coefficient (forces.py:237-276) of a (T, 3) force history:
+ * dL/dF from dL/dCd (T doubles). */
+int lbm_drag_coefficient_vjp(const double* force, const double* g_cd, int T,
+                             double inlet_velocity, const double* area, double rho,
+                             double* g_force, lbm_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -90,6 +90,27 @@ _SIGS = {
     "lbm_step_adjoint": (c_int, [ctypes.POINTER(StepDesc), ctypes.c_void_p, c_int, c_void_p, c_void_p, c_void_p,
                                  c_void_p, c_void_p]),
     "lbm_adjoint_scatter": (c_int, [ctypes.POINTER(StepDesc), ctypes.c_void_p, c_void_p, c_void_p, c_void_p]),
+    "lbm_equilibrium_vjp": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,
+                                    c_void_p]),
+    "lbm_density_vjp": (c_int, [c_int, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
+    "lbm_velocity_vjp": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,
+                                 c_void_p]),
+    "lbm_collide_vjp": (c_int, [c_int, c_int, c_void_p, c_double, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+                                c_int, c_int, c_void_p]),
+    "lbm_smagorinsky_tau_vjp": (c_int, [c_int, c_void_p, c_double, c_double, c_void_p, c_void_p, c_void_p, c_int,
+                                        c_int, c_int, c_void_p]),
+    "lbm_soft_bounce_back_vjp": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int,
+                                         c_int, c_void_p]),
+    "lbm_hard_bounce_vjp": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
+    "lbm_stream_vjp": (c_int, [c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,
+                               c_void_p]),
+    "lbm_zou_he_inlet_vjp": (c_int, [c_int, c_void_p, c_double, c_double, c_double, c_int, c_double, c_void_p,
+                                     c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
+    "lbm_zou_he_outlet_vjp": (c_int, [c_int, c_double, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
+    "lbm_momentum_exchange_vjp": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+                                          c_int, c_int, c_void_p]),
+    "lbm_drag_coefficient_vjp": (c_int, [c_void_p, c_void_p, c_int, c_double, c_void_p, c_double, c_void_p,
+                                         c_void_p]),
 }
 
 _lib = None

@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "liblbm_hip.so")
-SOURCES = ["lbm_ops.hip", "lbm_fused.hip", "lbm_adjoint.hip"]
+SOURCES = ["lbm_ops.hip", "lbm_ops_vjp.hip", "lbm_fused.hip", "lbm_adjoint.hip"]
 ARCH = os.environ.get("LBM_OFFLOAD_ARCH", "gfx950")
 
 

@@ -173,7 +173,7 @@ def test_default_state_layouts(dev):
     lat = Lattice(256, 128, 128, tau=0.6, device=dev)
     assert lat.layout == "rows" and lat.P == 256 and lat.RS == 19 * 256
     buf = lat.new_state()
-    assert buf.stride() == (256, 128 * 19 * 256, 19 * 256, 1) and buf.storage().size() == 19 * lat.N
+    assert buf.stride() == (256, 128 * 19 * 256, 19 * 256, 1) and buf.untyped_storage().nbytes() == 19 * lat.N * 4
     lat = Lattice(256, 128, 128, tau=0.6, device=dev, layout="apart")
     assert lat.P > lat.N and lat.P % 8192 == 1024
     buf = lat.new_state()
