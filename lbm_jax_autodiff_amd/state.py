@@ -4,12 +4,18 @@
 hard bounce (solid >= 1) -> stream with Bouzidi -> Zou-He inlet -> Zou-He
 outlet -> post-step stability guard (state.py:129-171) — as two launches of
 the fused kernel (collide, then stream).  ``simulate`` runs n steps with one
-launch per step.  ``checkpoint`` only affects reverse-mode memory in the
-reference and is accepted for signature compatibility.
+launch per step.
+
+When the populations, q or a tensor inlet velocity require grad, both run
+through the differentiable fused rollout (hand-written adjoint), with
+periodic y/z as the adjoint requires.  ``checkpoint`` then picks its memory
+policy, as jax.checkpoint does in the reference: True recomputes segments of
+about sqrt(n) steps, False keeps every state.
 """
 
 from __future__ import annotations
 
+import math
 from typing import NamedTuple
 
 import torch
@@ -51,9 +57,37 @@ def _lattice(state: LBMState, params: LBMParams) -> Lattice:
                    storage=storage, device=state.f.device)
 
 
+def _wants_grad(state: LBMState, params: LBMParams) -> bool:
+    if not torch.is_grad_enabled():
+        return False
+    vel = params.inlet_velocity
+    return bool(state.f.requires_grad or (state.q is not None and state.q.requires_grad)
+                or (isinstance(vel, torch.Tensor) and vel.requires_grad))
+
+
+def _rollout(state: LBMState, params: LBMParams, n: int, checkpoint: bool) -> torch.Tensor:
+    from .adjoint import rollout
+    if not bool(params.periodic_yz):
+        raise NotImplementedError("reverse mode of step/simulate needs periodic_yz=True")
+    if isinstance(params.tau, torch.Tensor) and params.tau.requires_grad:
+        raise NotImplementedError("reverse mode of step/simulate does not differentiate tau")
+    vel = params.inlet_velocity
+    if not isinstance(vel, torch.Tensor):
+        vel = [float(v) for v in vel]
+    policy = max(1, int(math.ceil(math.sqrt(n)))) if checkpoint and n > 1 else "all"
+    storage = "f64" if state.f.dtype == torch.float64 else "f32"
+    S, _, _ = rollout(state.f, n, tau=F.scalar_float(params.tau), q=state.q, solid=state.solid,
+                      inlet_velocity=vel, collision="mrt" if int(params.collision_mode) == 1 else "bgk",
+                      smagorinsky_cs=float(params.smagorinsky_cs) if int(params.use_smagorinsky) else None,
+                      guard="post", storage=storage, checkpoint=policy)
+    return S
+
+
 def step(state: LBMState, params: LBMParams) -> LBMState:
     """One timestep.  jax_lbm/state.py:87-171."""
     F.require_cuda(state.f, "state.f")
+    if _wants_grad(state, params):
+        return LBMState(_rollout(state, params, 1, False), state.solid, state.q)
     lat = _lattice(state, params)
     s_final, _, _ = lat.run(state.f, 1, forces=False)
     return LBMState(s_final, state.solid, state.q)
@@ -64,6 +98,8 @@ def simulate(state: LBMState, params: LBMParams, n_steps: int, checkpoint: bool 
     F.require_cuda(state.f, "state.f")
     if int(n_steps) == 0:
         return state
+    if _wants_grad(state, params):
+        return LBMState(_rollout(state, params, int(n_steps), checkpoint), state.solid, state.q)
     lat = _lattice(state, params)
     s_final, _, _ = lat.run(state.f, int(n_steps), forces=False)
     return LBMState(s_final, state.solid, state.q)

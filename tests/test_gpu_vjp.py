@@ -203,3 +203,29 @@ def test_kat_bouzidi_radius_gradient_per_op(dev):
     cd = lbm.drag_coefficient(force, u_in, area)
     (g,) = torch.autograd.grad(cd, r)
     assert abs(float(g) - KAT["d_cd_d_r_bouzidi_soft"]["value"]) < 5e-5, float(g)
+
+
+@pytest.mark.parametrize("checkpoint", [True, False])
+@pytest.mark.parametrize("collision_mode", [0, 1])
+def test_simulate_reverse_mode(dev, checkpoint, collision_mode):
+    """simulate() differentiates through the fused rollout: gradients with
+    respect to f0, q and the inlet velocity match torch autograd of the
+    oracle's simulate (state.py:174-230, post-step guard, hard bounce)."""
+    f0 = near_eq_state(NX, NY, NZ, amp=2e-3)
+    q = random_q(NX, NY, NZ, seed=9, p_wall=0.12)
+    g = torch.Generator().manual_seed(10)
+    solid = (torch.rand((NX, NY, NZ), generator=g) < 0.08).to(torch.int32)
+    vel = torch.tensor([0.05, 0.0, 0.0], dtype=DT)
+    n = 7
+
+    def ref(f, qq, v):
+        st = O.simulate(O.LBMState(f, solid, qq), O.LBMParams(torch.tensor(0.57, dtype=DT), v, collision_mode),
+                        n)
+        return st.f
+
+    def gpu(f, qq, v):
+        st = lbm.simulate(lbm.LBMState(f, solid.to(dev), qq), lbm.LBMParams(0.57, v, collision_mode), n,
+                          checkpoint=checkpoint)
+        return st.f
+
+    _check(dev, ref, gpu, [(f0, True), (q, True), (vel, True)], tol=1e-11)
