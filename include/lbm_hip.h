@@ -401,6 +401,19 @@ int lbm_momentum_exchange_vjp(int dtype, const void* f_post, const void* q, cons
                               const double* g_pressure, void* g_post, void* g_streamed, int nx,
                               int ny, int nz, lbm_stream_t stream);
 
+/* Sphere geometry (jax_lbm/geometry/smooth.py:158-248 for mode 1,
+ * primitives.py:56-174 for mode 0; same arguments as lbm_sphere_geometry,
+ * solid = its output): out[4] = dL/d(radius, cx, cy, cz) in float64 from the
+ * cotangents of the porosity (N, optional) and of q (19 x N, optional).
+ * The solid mask and the link set carry no cotangent (comparisons).
+ * workspace: lbm_sphere_geometry_vjp_workspace_bytes() bytes; fixed-order
+ * reduction. */
+size_t lbm_sphere_geometry_vjp_workspace_bytes(void);
+int lbm_sphere_geometry_vjp(int mode, int coord_dtype, int out_dtype, double cx, double cy, double cz,
+                            double radius, double sharpness, double disc_floor, double denom_eps,
+                            const int32_t* solid, const void* g_porosity, const void* g_q, double* out,
+                            double* workspace, int nx, int ny, int nz, lbm_stream_t stream);
+
 /* drag_coefficient (forces.py:237-276) of a (T, 3) force history:
  * dL/dF from dL/dCd (T doubles). */
 int lbm_drag_coefficient_vjp(const double* force, const double* g_cd, int T,

@@ -111,6 +111,10 @@ _SIGS = {
                                           c_int, c_int, c_void_p]),
     "lbm_drag_coefficient_vjp": (c_int, [c_void_p, c_void_p, c_int, c_double, c_void_p, c_double, c_void_p,
                                          c_void_p]),
+    "lbm_sphere_geometry_vjp_workspace_bytes": (c_size_t, []),
+    "lbm_sphere_geometry_vjp": (c_int, [c_int, c_int, c_int, c_double, c_double, c_double, c_double, c_double,
+                                        c_double, c_double, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+                                        c_int, c_int, c_void_p]),
 }
 
 _lib = None

@@ -1,6 +1,6 @@
 // Per-operator kernels behind the jax_lbm.core-compatible surface.
 // One thread per cell; populations are read with stride N (coalesced along x).
-#include "lbm_physics.hpp"
+#include "lbm_geometry.hpp"
 
 namespace lbm {
 
@@ -311,15 +311,6 @@ __global__ void k_drag(const double* __restrict__ force, int T, double U, const 
 }
 
 // ---- geometry --------------------------------------------------------------
-template <typename C>
-__device__ __forceinline__ void world_xyz(int x, int y, int z, int nx, int ny, int nz, C& wx, C& wy, C& wz) {
-  // node (i,j,k) -> ((i+1/2)/(nx/8) - 4, (j+1/2)/(ny/4) - 2, (k+1/2)/(nz/4) - 2)
-  // (primitives.py:26-53): same operation order and precision as the reference
-  wx = (C(x) + C(0.5)) / C(nx / 8.0) - C(4.0);
-  wy = (C(y) + C(0.5)) / C(ny / 4.0) - C(2.0);
-  wz = (C(z) + C(0.5)) / C(nz / 4.0) - C(2.0);
-}
-
 template <typename C, typename O>
 __global__ void k_sphere_cells(int mode, C cx, C cy, C cz, C r, C alpha, int32_t* __restrict__ solid,
                                O* __restrict__ porosity, O* __restrict__ dphidr, int nx, int ny, int nz) {
@@ -361,15 +352,6 @@ __global__ void k_sigmoid(const R* __restrict__ sdf, R alpha, R* __restrict__ ou
   const size_t c = blockIdx.x * size_t(blockDim.x) + threadIdx.x;
   if (c >= N) return;
   out[c] = R(1) / (R(1) + exp(-(alpha * sdf[c])));
-}
-
-__device__ __forceinline__ bool link_from(const int32_t* solid, int x, int y, int z, int i, int nx, int ny,
-                                          int nz) {
-  // fluid cell whose in-bounds neighbour along e_i is a body cell
-  // (primitives.py:95-113, bouzidi.py:46-66, smooth.py:168-185)
-  const int bx = x + EX[i], by = y + EY[i], bz = z + EZ[i];
-  if (bx < 0 || bx >= nx || by < 0 || by >= ny || bz < 0 || bz >= nz) return false;
-  return solid[(size_t(z) * ny + y) * nx + x] == 0 && solid[(size_t(bz) * ny + by) * nx + bx] == 1;
 }
 
 template <typename C, typename O>

@@ -9,6 +9,7 @@
 // the cotangent to the taken branch, `clip` passes it inside its bounds only,
 // comparisons carry none.
 #include "lbm_adjoint_physics.hpp"
+#include "lbm_geometry.hpp"
 
 namespace lbm {
 
@@ -389,6 +390,87 @@ __global__ void k_drag_vjp(const double* __restrict__ force, const double* __res
   gforce[3 * t + 2] = 0.0;
 }
 
+// Sphere geometry (jax_lbm/geometry/smooth.py:158-248, primitives.py:56-174):
+// dL/d(r, cx, cy, cz) from the cotangents of the porosity and of q, in
+// float64, per workgroup (grid-stride, fixed order) into part[blockIdx][4].
+//   porosity = sigmoid(alpha (r - |x - c|)):  d/dr = alpha p (1 - p),
+//     d/dc = alpha p (1 - p) (x - c) / |x - c|  (0 at |x - c| = 0, where JAX gives NaN);
+//   q = clip(t, 0.01, 0.99) on links with t = (-b - sqrt(max(disc, floor))) / (2a + eps),
+//     b = 2 o.d, cc = |o|^2 - r^2, disc = b^2 - 4 a cc, o = x - c, d = e_i / s:
+//     dt/dr = -(4 a r / sqrt(disc)) / (2a + eps),
+//     dt/dc = (2 d + (2 b d - 4 a o) / sqrt(disc)) / (2a + eps),
+//   only where the ray hit is valid, disc > floor and 0.01 < t < 0.99
+//   (the where, max and clip then pass the cotangent), else 0.
+template <typename C, typename O>
+__global__ void k_sphere_vjp(int mode, C cx, C cy, C cz, C r, C alpha, C disc_floor, C denom_eps,
+                             const int32_t* __restrict__ solid, const O* __restrict__ gphi, const O* __restrict__ gq,
+                             double* __restrict__ part, int nx, int ny, int nz) {
+  const size_t N = size_t(nx) * ny * nz;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (size_t c = blockIdx.x * size_t(blockDim.x) + threadIdx.x; c < N; c += size_t(gridDim.x) * blockDim.x) {
+    const int x = int(c % nx), y = int((c / nx) % ny), z = int(c / (size_t(nx) * ny));
+    C wx, wy, wz;
+    world_xyz(x, y, z, nx, ny, nz, wx, wy, wz);
+    const C ox = wx - cx, oy = wy - cy, oz = wz - cz;
+    if (gphi && mode == 1) {
+      const C dist = sqrt(ox * ox + oy * oy + oz * oz);
+      const C phi = C(1) / (C(1) + exp(-(alpha * (r - dist))));
+      const C k = C(gphi[c]) * alpha * phi * (C(1) - phi);
+      acc[0] += double(k);
+      if (dist > C(0)) {
+        acc[1] += double(k * ox / dist);
+        acc[2] += double(k * oy / dist);
+        acc[3] += double(k * oz / dist);
+      }
+    }
+    if (gq) {
+      for (int i = 1; i < Q; ++i) {
+        if (!link_from(solid, x, y, z, i, nx, ny, nz)) continue;
+        const C g = C(gq[i * N + c]);
+        if (g == C(0)) continue;
+        const C dx = C(EX[i]) / C(nx / 8.0), dy = C(EY[i]) / C(ny / 4.0), dz = C(EZ[i]) / C(nz / 4.0);
+        const C a = dx * dx + dy * dy + dz * dz;
+        const C b = C(2) * (ox * dx + oy * dy + oz * dz);
+        const C cc = ox * ox + oy * oy + oz * oz - r * r;
+        const C disc = b * b - C(4) * a * cc;
+        const C sq = sqrt(fmax(disc, disc_floor));
+        const C den = C(2) * a + denom_eps;
+        const C t = (-b - sq) / den;
+        const bool valid = (disc >= C(0)) && (t > C(0)) && (t <= C(1));
+        if (!(valid && disc > disc_floor && t > C(0.01) && t < C(0.99))) continue;
+        acc[0] += double(g * (-(C(4) * a * r / sq) / den));
+        acc[1] += double(g * (C(2) * dx + (C(2) * b * dx - C(4) * a * ox) / sq) / den);
+        acc[2] += double(g * (C(2) * dy + (C(2) * b * dy - C(4) * a * oy) / sq) / den);
+        acc[3] += double(g * (C(2) * dz + (C(2) * b * dz - C(4) * a * oz) / sq) / den);
+      }
+    }
+  }
+  __shared__ double sh[4][kVjpBlock / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double v = wave_sum(acc[k]);
+    if (lane == 0) sh[k][wid] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    double s = 0.0;
+    for (int w = 0; w < kVjpBlock / 64; ++w) s += sh[threadIdx.x][w];
+    part[blockIdx.x * 4 + threadIdx.x] = s;
+  }
+}
+
+// out[k] = sum over the rows of part[rows][4], one thread per k, fixed order.
+__global__ void k_sum4(const double* __restrict__ part, int rows, double* __restrict__ out) {
+  const int k = threadIdx.x;
+  if (k >= 4) return;
+  double s = 0.0;
+  for (int r = 0; r < rows; ++r) s += part[r * 4 + k];
+  out[k] = s;
+}
+
+constexpr int kGeomVjpBlocks = 512;
+
 }  // namespace lbm
 
 using namespace lbm;
@@ -521,6 +603,28 @@ int lbm_momentum_exchange_vjp(int dtype, const void* f_post, const void* q, cons
   LBM_VJP_DISPATCH(dtype, hipLaunchKernelGGL(k_mex_vjp<R>, vjp_grid(N), kVjpBlock, 0, (hipStream_t)stream,
                                              (const R*)f_post, (const R*)q, g_force, g_pressure, (R*)g_post,
                                              (R*)g_streamed, N));
+  return vjp_status();
+}
+
+size_t lbm_sphere_geometry_vjp_workspace_bytes(void) { return size_t(kGeomVjpBlocks) * 4 * sizeof(double); }
+
+int lbm_sphere_geometry_vjp(int mode, int coord_dtype, int out_dtype, double cx, double cy, double cz, double radius,
+                            double sharpness, double disc_floor, double denom_eps, const int32_t* solid,
+                            const void* g_porosity, const void* g_q, double* out, double* workspace, int nx, int ny,
+                            int nz, lbm_stream_t stream) {
+  if (!vjp_dims_ok(nx, ny, nz) || !solid || !out || !workspace || (mode != 0 && mode != 1)) return LBM_E_ARG;
+  hipStream_t s = (hipStream_t)stream;
+#define LBM_GEOM_VJP(C, O)                                                                                          \
+  hipLaunchKernelGGL((k_sphere_vjp<C, O>), kGeomVjpBlocks, kVjpBlock, 0, s, mode, C(cx), C(cy), C(cz), C(radius),   \
+                     C(sharpness), C(mode == 0 ? 0.0 : disc_floor), C(denom_eps), solid, (const O*)g_porosity,       \
+                     (const O*)g_q, workspace, nx, ny, nz)
+  if (coord_dtype == LBM_F32 && out_dtype == LBM_F32) { LBM_GEOM_VJP(float, float); }
+  else if (coord_dtype == LBM_F32 && out_dtype == LBM_F64) { LBM_GEOM_VJP(float, double); }
+  else if (coord_dtype == LBM_F64 && out_dtype == LBM_F32) { LBM_GEOM_VJP(double, float); }
+  else if (coord_dtype == LBM_F64 && out_dtype == LBM_F64) { LBM_GEOM_VJP(double, double); }
+  else return LBM_E_DTYPE;
+#undef LBM_GEOM_VJP
+  hipLaunchKernelGGL(k_sum4, 1, 64, 0, s, workspace, kGeomVjpBlocks, out);
   return vjp_status();
 }
 

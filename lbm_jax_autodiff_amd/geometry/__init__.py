@@ -63,8 +63,18 @@ def dot64(a: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
 
 def create_sphere(nx, ny, nz, center, radius, device=None):
     """Hard sphere (dist^2 <= r^2) and ray-sphere Bouzidi q.
-    jax_lbm/geometry/primitives.py:56-174.  Returns (solid int32, q float32)."""
-    solid, _, q, _, _ = sphere_geometry_raw(nx, ny, nz, center, radius, mode=0, device=device)
+    jax_lbm/geometry/primitives.py:56-174.  Returns (solid int32, q float32);
+    q is differentiable in ``radius`` and ``center`` when they require grad."""
+    dev = device
+    for v in (radius, center):
+        if isinstance(v, torch.Tensor) and v.is_cuda:
+            dev = v.device
+    dev = _device(dev)
+    if _needs_grad(radius) or _needs_grad(center):
+        _, solid, q = _SphereGeometry.apply(radius, center, nx, ny, nz, 0, 0.0, torch.float32, torch.float32,
+                                            0.0, 1e-20, dev)
+        return solid, q
+    solid, _, q, _, _ = sphere_geometry_raw(nx, ny, nz, _center_floats(center), radius, mode=0, device=dev)
     return F.scalar_view(solid), F.pop_view(q)
 
 
@@ -101,28 +111,58 @@ def porosity_from_sdf(sdf: torch.Tensor, sharpness: float = 20.0) -> torch.Tenso
     return F.scalar_view(out)
 
 
-class _SmoothSphere(torch.autograd.Function):
-    """(porosity, q) as differentiable functions of the radius."""
+def _center_floats(center):
+    if isinstance(center, torch.Tensor):
+        vals = center.detach().reshape(-1).to("cpu", torch.float64).tolist()
+    else:
+        vals = [float(c) for c in center]
+    if len(vals) != 3:
+        raise ValueError("center must have 3 components")
+    return tuple(vals)
+
+
+class _SphereGeometry(torch.autograd.Function):
+    """(porosity, solid, q) of a sphere as differentiable functions of the
+    radius and the centre; the backward is one fixed-order reduction kernel
+    (lbm_sphere_geometry_vjp), so no per-cell derivative fields are stored."""
 
     @staticmethod
-    def forward(ctx, radius, nx, ny, nz, center, sharpness, coord_dtype, out_dtype, disc_floor, denom_eps, device):
-        solid, por, q, dqdr, dphidr = sphere_geometry_raw(
-            nx, ny, nz, center, radius, mode=1, sharpness=sharpness, coord_dtype=coord_dtype, out_dtype=out_dtype,
-            disc_floor=disc_floor, denom_eps=denom_eps, with_grads=True, device=device)
-        ctx.save_for_backward(dqdr, dphidr)
-        ctx.r_dtype = radius.dtype
+    def forward(ctx, radius, center, nx, ny, nz, mode, sharpness, coord_dtype, out_dtype, disc_floor, denom_eps,
+                device):
+        c = _center_floats(center)
+        solid, por, q, _, _ = sphere_geometry_raw(nx, ny, nz, c, radius, mode=mode, sharpness=sharpness,
+                                                  coord_dtype=coord_dtype, out_dtype=out_dtype,
+                                                  disc_floor=disc_floor, denom_eps=denom_eps, device=device)
+        ctx.save_for_backward(solid)
+        ctx.cfg = (nx, ny, nz, mode, sharpness, coord_dtype, out_dtype, disc_floor, denom_eps, c,
+                   F.scalar_float(radius))
+        ctx.r_like, ctx.c_like = radius, center
         ctx.mark_non_differentiable(solid)
         return F.scalar_view(por), F.scalar_view(solid), F.pop_view(q)
 
     @staticmethod
     def backward(ctx, g_por, g_solid, g_q):
-        dqdr, dphidr = ctx.saved_tensors
-        total = torch.zeros((), dtype=torch.float64, device=dqdr.device)
-        if g_por is not None:
-            total = total + dot64(F.scalar_storage(g_por), dphidr)
-        if g_q is not None:
-            total = total + dot64(F.pop_storage(g_q), dqdr)
-        return (total.to(ctx.r_dtype),) + (None,) * 10
+        (solid,) = ctx.saved_tensors
+        nx, ny, nz, mode, sharpness, coord_dtype, out_dtype, disc_floor, denom_eps, c, r = ctx.cfg
+        dev = solid.device
+        gp = F.scalar_storage(g_por.to(out_dtype)) if g_por is not None else None
+        gq = F.pop_storage(g_q.to(out_dtype)) if g_q is not None else None
+        out = torch.empty(4, dtype=torch.float64, device=dev)
+        ws = torch.empty(int(L.load().lbm_sphere_geometry_vjp_workspace_bytes()) // 8, dtype=torch.float64,
+                         device=dev)
+        L.call("lbm_sphere_geometry_vjp", int(mode), _tag(coord_dtype), _tag(out_dtype), c[0], c[1], c[2], r,
+               float(sharpness), float(disc_floor), float(denom_eps), L.ptr(solid), L.ptr(gp), L.ptr(gq),
+               L.ptr(out), L.ptr(ws), nx, ny, nz, L.stream_handle(dev))
+        g_r = g_c = None
+        if isinstance(ctx.r_like, torch.Tensor) and ctx.r_like.requires_grad:
+            g_r = out[0].reshape(ctx.r_like.shape).to(ctx.r_like.dtype).to(ctx.r_like.device)
+        if isinstance(ctx.c_like, torch.Tensor) and ctx.c_like.requires_grad:
+            g_c = out[1:].reshape(ctx.c_like.shape).to(ctx.c_like.dtype).to(ctx.c_like.device)
+        return (g_r, g_c) + (None,) * 10
+
+
+def _needs_grad(v) -> bool:
+    return isinstance(v, torch.Tensor) and v.requires_grad and torch.is_grad_enabled()
 
 
 def smooth_sphere_geometry(nx, ny, nz, center, radius, sharpness: float = 20.0, *, coord_dtype=torch.float32,
@@ -130,17 +170,19 @@ def smooth_sphere_geometry(nx, ny, nz, center, radius, sharpness: float = 20.0, 
     """(porosity, solid, q) with porosity = sigmoid(sharpness (r - dist)),
     solid = porosity > 1/2 and ray-sphere q(r).  jax_lbm/geometry/smooth.py:158-248.
 
-    If ``radius`` is a tensor that requires grad, porosity and q carry the
-    derivative with respect to it (dq/dr = 0 where q is clipped or invalid,
-    as JAX's clip/where give).
+    ``radius`` and ``center`` (a (3,) tensor) may require grad: porosity and q
+    then carry their derivatives (dq = 0 where q is clipped or invalid, as
+    JAX's clip/where give; the mask and the link set carry none).
     """
-    if isinstance(radius, torch.Tensor):
-        dev = radius.device if radius.is_cuda else _device(device)
-        r = radius if radius.is_cuda else radius.to(dev)
-        if r.requires_grad:
-            return _SmoothSphere.apply(r, nx, ny, nz, tuple(center), float(sharpness), coord_dtype, out_dtype,
-                                       float(disc_floor), float(denom_eps), dev)
-    solid, por, q, _, _ = sphere_geometry_raw(nx, ny, nz, center, radius, mode=1, sharpness=sharpness,
-                                              coord_dtype=coord_dtype, out_dtype=out_dtype, disc_floor=disc_floor,
-                                              denom_eps=denom_eps, device=device)
+    dev = device
+    for v in (radius, center):
+        if isinstance(v, torch.Tensor) and v.is_cuda:
+            dev = v.device
+    dev = _device(dev)
+    if _needs_grad(radius) or _needs_grad(center):
+        return _SphereGeometry.apply(radius, center, nx, ny, nz, 1, float(sharpness), coord_dtype, out_dtype,
+                                     float(disc_floor), float(denom_eps), dev)
+    solid, por, q, _, _ = sphere_geometry_raw(nx, ny, nz, _center_floats(center), radius, mode=1,
+                                              sharpness=sharpness, coord_dtype=coord_dtype, out_dtype=out_dtype,
+                                              disc_floor=disc_floor, denom_eps=denom_eps, device=dev)
     return F.scalar_view(por), F.scalar_view(solid), F.pop_view(q)

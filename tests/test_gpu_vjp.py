@@ -229,3 +229,57 @@ def test_simulate_reverse_mode(dev, checkpoint, collision_mode):
         return st.f
 
     _check(dev, ref, gpu, [(f0, True), (q, True), (vel, True)], tol=1e-11)
+
+
+def _geometry_grads(geom, nx, ny, nz, center, radius, dev, weights_seed=12, **kw):
+    """d/d(r, c) of <w_p, porosity> + <w_q, q> (and the outputs) for a geometry function."""
+    c = torch.tensor(center, dtype=DT, device=dev, requires_grad=True)
+    r = torch.tensor(radius, dtype=DT, device=dev, requires_grad=True)
+    out = geom(nx, ny, nz, c, r, **kw)
+    por, q = (out[0], out[2]) if len(out) == 3 else (None, out[1])
+    g = torch.Generator().manual_seed(weights_seed)
+    loss = (q * torch.randn(q.shape, generator=g, dtype=DT).to(q.device, q.dtype)).sum()
+    if por is not None:
+        loss = loss + (por * torch.randn(por.shape, generator=g, dtype=DT).to(por.device, por.dtype)).sum()
+    gr, gc = torch.autograd.grad(loss, (r, c))
+    return float(gr), [float(v) for v in gc]
+
+
+@pytest.mark.parametrize("center", [(-0.3, 0.1, -0.05), (0.0, 0.0, 0.0)])
+def test_smooth_sphere_center_and_radius_gradients(dev, center):
+    """dq/dr, dq/dc, dporosity/dr, dporosity/dc (smooth.py:158-248) against
+    torch autograd of the oracle, float64 coordinates."""
+    nx, ny, nz = 32, 16, 16
+
+    def ref(nx_, ny_, nz_, c, r):
+        return O.smooth_sphere_geometry(nx_, ny_, nz_, c, r, 10.0, coord_dtype=DT)
+
+    def gpu(nx_, ny_, nz_, c, r):
+        return lbm.smooth_sphere_geometry(nx_, ny_, nz_, c, r, 10.0, coord_dtype=DT, out_dtype=DT)
+
+    gr_ref, gc_ref = _geometry_grads(ref, nx, ny, nz, center, 0.55, torch.device("cpu"))
+    gr, gc = _geometry_grads(gpu, nx, ny, nz, center, 0.55, dev)
+    assert abs(gr - gr_ref) <= 1e-10 * max(1.0, abs(gr_ref)), (gr, gr_ref)
+    for a, b in zip(gc, gc_ref):
+        assert abs(a - b) <= 1e-10 * max(1.0, abs(b)), (gc, gc_ref)
+
+
+def test_create_sphere_q_gradients(dev):
+    """The hard sphere's ray q (primitives.py:56-174) in radius and centre,
+    float32 coordinates as in the reference, against oracle autograd."""
+    nx, ny, nz = 32, 16, 16
+    c0, r0 = (-0.3, 0.1, -0.05), 0.55
+
+    def grads(geom, device, dt):
+        c = torch.tensor(c0, dtype=dt, device=device, requires_grad=True)
+        r = torch.tensor(r0, dtype=dt, device=device, requires_grad=True)
+        solid, q = geom(c, r)
+        g = torch.Generator().manual_seed(13)
+        w = torch.randn(q.shape, generator=g, dtype=torch.float64).to(q.device, q.dtype)
+        return torch.autograd.grad((q * w).sum(), (r, c)), solid
+
+    (gr_ref, gc_ref), s_ref = grads(lambda c, r: O.create_sphere(nx, ny, nz, c, r), "cpu", torch.float32)
+    (gr, gc), s = grads(lambda c, r: lbm.create_sphere(nx, ny, nz, c, r, device=dev), dev, torch.float32)
+    assert torch.equal(cpu(s), s_ref)
+    assert abs(float(gr) - float(gr_ref)) <= 1e-4 * abs(float(gr_ref))
+    assert float((cpu(gc) - gc_ref).abs().max()) <= 1e-4 * float(gc_ref.abs().max())
