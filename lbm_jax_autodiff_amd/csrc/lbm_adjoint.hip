@@ -352,7 +352,8 @@ __global__ __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(adj_w
     adjoint_cell<S, LM, COLL, MODE, true>(a, P, lam_in, lam_out, gF, c, b, a.flags[c], qv, dsdq[threadIdx.x]);
     return;
   }
-  const uint32_t tile = xcd_swizzle(bid - uint32_t(a.nlb), uint32_t(a.ndb));
+  // tiles in workgroup order: consecutive rows run on all XCDs at once
+  const uint32_t tile = bid - uint32_t(a.nlb);
   const uint32_t c = tile * kAdjBlock + threadIdx.x;
   if (c >= a.N) return;
   adjoint_cell<S, LM, COLL, MODE, false>(a, P, lam_in, lam_out, gF, c, -1, a.flags ? uint32_t(a.flags[c]) : 0u,

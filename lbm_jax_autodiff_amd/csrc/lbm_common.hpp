@@ -268,17 +268,6 @@ inline bool state_layout(const lbm_step_desc* d, long long& pstride, long long& 
   return extent < (1LL << 31);
 }
 
-// Map a hardware workgroup id to a logical tile so that consecutive logical
-// tiles land on the same XCD (workgroups are dealt round-robin to the 8 XCDs
-// of an MI355X).  Keeps the halo cache lines of neighbouring tiles in one L2.
-__device__ __forceinline__ uint32_t xcd_swizzle(uint32_t bid, uint32_t nblocks) {
-  constexpr uint32_t NX = 8;
-  const uint32_t per = nblocks / NX;
-  const uint32_t full = per * NX;
-  if (bid >= full) return bid;
-  return (bid % NX) * per + bid / NX;
-}
-
 template <typename R>
 __device__ __forceinline__ R wave_sum(R v) {
 #pragma unroll

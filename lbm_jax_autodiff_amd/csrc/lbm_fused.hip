@@ -9,9 +9,12 @@
 //
 // One launch per step.  Its first workgroups own the sparse link cells
 // (fluid cells with a Bouzidi link, listed in link_cell/link_q) and produce
-// the float64 momentum-exchange partials; the rest sweep the dense grid and
-// skip link cells, re-mapped so that consecutive x-rows share an XCD (and its
-// L2).  The link path is written to fit the dense sweep's register budget
+// the float64 momentum-exchange partials; the rest sweep the dense grid in
+// workgroup order and skip link cells.  Hardware dispatch deals consecutive
+// workgroups to the 8 XCDs in turn, so at any moment all XCDs update
+// neighbouring rows and the row-interleaved state is read and written in a
+// narrow address window (remapping tiles so that each XCD sweeps its own
+// contiguous eighth of the grid was 6% slower).  The link path is written to fit the dense sweep's register budget
 // (link_cell_update): a link path that needs more either lowers the occupancy
 // of every workgroup or, held to the budget, spills and finishes last under
 // the full memory load; as a separate launch ahead of the sweep it cost
@@ -411,7 +414,7 @@ __global__ __launch_bounds__(kStepBlock) __attribute__((amdgpu_waves_per_eu(step
 
   // ---------------- dense sweep ----------------
   const uint32_t first = (MODE == LBM_MODE_COLLIDE) ? 0u : uint32_t(a.nlb);
-  const uint32_t tile = xcd_swizzle(bid - first, uint32_t(a.ndb));
+  const uint32_t tile = bid - first;
   if constexpr (V > 1) {
     dense_vec<S, COLL, MODE, V>(a, src, dst, tile * kStepBlock + threadIdx.x);
     return;
