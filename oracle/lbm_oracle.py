@@ -12,7 +12,8 @@ It restates, in PyTorch on the CPU (float32 or float64, differentiable with
 follows.  The layout is the reference's: populations ``f[i, x, y, z]`` of shape
 ``(19, nx, ny, nz)``.
 
-How each part is pinned (tests in ``tests/test_oracle_*.py``):
+How each part is pinned (tests in ``tests/test_oracle.py``; the published
+values are read from ``tests/golden/kats.json``):
 
 * lattice tables, MRT basis     closed forms (weight moments, M M^T = diag(norms)),
                                  the reference's row-norm table
