@@ -225,10 +225,13 @@ typedef struct lbm_step_desc {
   const void* porosity;     /* N, float (double for LBM_F64): soft bounce */
   const int32_t* link_cell; /* n_links */
   const double* link_q;     /* 18 x n_links */
-  int32_t cells_per_thread; /* dense sweep: 1, 2 or 4 x-consecutive cells per
-                               thread; 2 and 4 apply when they divide nx
-                               and pop_stride, anything else (0 = default)
-                               means 1 */
+  int32_t cells_per_thread; /* dense sweep: 0 = automatic (one cell per
+                               thread; where nx % 64 == 0 each wave's row
+                               addresses are scalar -- the adjoint, and the
+                               forward with 16-bit populations), 1 = one cell
+                               per thread with per-lane addresses, 2 or 4 =
+                               x-consecutive cells per thread when they divide
+                               nx and pop_stride (else 1) */
   int64_t pop_stride;       /* elements between population i and i+1 of every
                                state and cotangent buffer the fused entry
                                points read or write; 0 = N */

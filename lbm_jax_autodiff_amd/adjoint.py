@@ -199,7 +199,8 @@ class _RolloutFn(torch.autograd.Function):
                       porosity=porosity.detach() if porosity is not None else None,
                       q=q.detach() if q is not None else None, guard=cfg["guard"],
                       guard_velocity=cfg["guard_velocity"], periodic_yz=True, storage=cfg["storage"],
-                      offset=cfg["offset"], device=f0.device, layout=cfg["layout"])
+                      offset=cfg["offset"], device=f0.device, layout=cfg["layout"],
+                      cells_per_thread=cfg["cells_per_thread"])
         ck = _Checkpoints(lat, cfg["checkpoint"], n_steps)
         s0 = lat.to_state(f0.detach())
         a = lat.new_state()
@@ -350,7 +351,7 @@ def rollout(f0: torch.Tensor, n_steps: int, *, tau, q: Optional[torch.Tensor] = 
             collision: str = "bgk", smagorinsky_cs: Optional[float] = None, guard: str = "none",
             guard_velocity=None, storage: str = "f32", offset="weights",
             checkpoint: Union[str, int, Sequence[int]] = "all", adjoint_storage: Optional[str] = None,
-            layout: str = "rows"):
+            layout: str = "rows", cells_per_thread: int = 0):
     """Differentiable fused rollout: (S_N, P_{N-1}, forces[N, 3] float64).
 
     Step order as in the reference's drivers: [pre guard] -> [Smagorinsky]
@@ -359,7 +360,8 @@ def rollout(f0: torch.Tensor, n_steps: int, *, tau, q: Optional[torch.Tensor] = 
     ``adjoint_storage`` "f32" keeps the cotangents in the arithmetic type,
     "f16" stores them as scaled IEEE half (half storages only; default for
     them), halving the adjoint's cotangent traffic.  ``layout`` places the
-    state and cotangent buffers ("rows" or "apart", as for ``Lattice``).
+    state and cotangent buffers ("rows" or "apart", as for ``Lattice``);
+    ``cells_per_thread`` selects the sweeps' addressing as for ``Lattice``.
     """
     nx, ny, nz = F.dims_of_pop(f0)
     if isinstance(inlet_velocity, torch.Tensor):
@@ -373,7 +375,7 @@ def rollout(f0: torch.Tensor, n_steps: int, *, tau, q: Optional[torch.Tensor] = 
     cfg = dict(n_steps=int(n_steps), nx=nx, ny=ny, nz=nz, tau=F.scalar_float(tau), rho_out=float(rho_out),
                collision=collision, smagorinsky_cs=smagorinsky_cs, solid=solid, guard=guard,
                guard_velocity=guard_velocity, storage=storage, offset=offset, checkpoint=checkpoint,
-               adjoint_storage=adjoint_storage, layout=layout)
+               adjoint_storage=adjoint_storage, layout=layout, cells_per_thread=int(cells_per_thread))
     if int(n_steps) < 1:
         raise ValueError("n_steps must be >= 1")
     return _RolloutFn.apply(f0, q, porosity, u, cfg)

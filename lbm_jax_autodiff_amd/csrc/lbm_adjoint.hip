@@ -44,6 +44,9 @@ struct AdjArgs {
   double* u_acc;
   // cotangent storage scale (half storage) and its inverse
   R lam_scale, lam_inv;
+  // extents in bytes of a state and of a cotangent buffer (below 2^32 when
+  // the row-uniform sweep is used)
+  uint32_t state_bytes, lam_bytes;
 };
 
 // Storage of the cotangent (lambda) buffers: the arithmetic type itself, or
@@ -55,11 +58,25 @@ struct LamPlain {
   using T = R;
   __device__ static R ld(const T* p, size_t k, R) { return p[k]; }
   __device__ static void st(T* p, size_t k, R v, R) { p[k] = v; }
+  __device__ static R bld(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, R) {
+    if constexpr (sizeof(R) == 4) return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, int(vo), int(so), 0));
+    else return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b64(r, int(vo), int(so), 0));
+  }
+  __device__ static void bst(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, R v, R) {
+    if constexpr (sizeof(R) == 4) __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, int(vo), int(so), 0);
+    else __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, int(vo), int(so), 0);
+  }
 };
 struct LamHalf {
   using T = unsigned short;
   __device__ static float ld(const T* p, size_t k, float inv_scale) { return f16_bits_to_float(p[k]) * inv_scale; }
   __device__ static void st(T* p, size_t k, float v, float scale) { p[k] = float_to_f16_bits(v * scale); }
+  __device__ static float bld(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, float inv_scale) {
+    return f16_bits_to_float(__builtin_amdgcn_raw_buffer_load_b16(r, int(vo), int(so), 0)) * inv_scale;
+  }
+  __device__ static void bst(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, float v, float scale) {
+    __builtin_amdgcn_raw_buffer_store_b16(float_to_f16_bits(v * scale), r, int(vo), int(so), 0);
+  }
 };
 
 template <typename R>
@@ -183,10 +200,10 @@ __device__ __forceinline__ void apply_zou_he(const AdjArgs<R>& a, R* s, int x) {
 
 // g <- (pre-guard, collide, bounce)^T g at the collision input s (the cell's
 // S_{t+1}); s is replaced by the pre-guard reset state where the guard fires.
-// Accumulates dL/dporosity unless `skip`.  fl = the cell's flags.  Returns
-// true when the pre guard fired.
+// Accumulates dL/dporosity (the term times gunit, g's unit) unless `skip`.
+// fl = the cell's flags.  Returns true when the pre guard fired.
 template <typename R, int COLL>
-__device__ __forceinline__ bool collide_transpose(const AdjArgs<R>& a, R* s, R* g, uint32_t c, uint32_t fl,
+__device__ __forceinline__ bool collide_transpose(const AdjArgs<R>& a, R* s, R* g, R gunit, uint32_t c, uint32_t fl,
                                                   bool skip) {
   Pairs<R> ps = to_pairs(s);
   Macro<R> m = macro(ps);
@@ -208,7 +225,7 @@ __device__ __forceinline__ bool collide_transpose(const AdjArgs<R>& a, R* s, R* 
     if constexpr (COLL == LBM_COLL_MRT) collide_mrt(cc, omega, ps, m);
     else collide_bgk(cc, omega, m);
     const R gp = blend_vjp(cc, g, a.phi[c]);
-    if (a.phi_acc && !skip) a.phi_acc[c] += double(gp);
+    if (a.phi_acc && !skip) a.phi_acc[c] += double(gp) * double(gunit);
   }
   if (pre_reset) {
     sfor<Q>([&](auto I) {
@@ -222,45 +239,29 @@ __device__ __forceinline__ bool collide_transpose(const AdjArgs<R>& a, R* s, R* 
   return pre_reset;
 }
 
-// One cell: forward recompute + cotangent pull-back + owner writes.
-// MODE FULL: lam_in = dL/dP_{t+1}; MODE STREAM: lam_in = dL/dS_{t+1}.
-template <class S, class LM, int COLL, int MODE, bool LINK>
-__device__ __forceinline__ void adjoint_cell(const AdjArgs<typename S::R>& a, const typename S::T* __restrict__ P,
-                                             const typename LM::T* __restrict__ lam_in,
-                                             typename LM::T* __restrict__ lam_out, const double* __restrict__ gF,
-                                             uint32_t c, int b, uint32_t fl, const typename S::R* qv,
-                                             typename S::R* dsdq) {
+// The part of a cell's adjoint shared by the link, dense and row-uniform
+// sweeps: Zou-He and the post guard on the pulled s, the transposed
+// collision, the link cell's force seed, and the Zou-He transposes.  On entry
+// g = dL/dP_{t+1}(x) (MODE FULL) or dL/dS_{t+1}(x) (MODE STREAM); on exit
+// g = mu, the cotangent of the pulled values.  load_g(g) loads the incoming
+// cotangent; it runs after the forward boundaries, which keeps the 19 values
+// out of the registers while those are computed.  g may hold the cotangent
+// times a constant (a storage scale): the map is linear in g, and gunit
+// (1 / that constant) converts the accumulated parameter terms; the force
+// seed (link cells) needs gunit = 1.
+template <class S, int COLL, int MODE, bool LINK, class LoadG>
+__device__ __forceinline__ void cell_transpose(const AdjArgs<typename S::R>& a, typename S::R* s, typename S::R* g,
+                                               typename S::R gunit, LoadG&& load_g, uint32_t c, int x, uint32_t fl,
+                                               bool skip, uint32_t lmask, const double* __restrict__ gF) {
   using R = typename S::R;
-  const size_t N = a.P;
-  // a dense-sweep thread on a link cell computes but writes nothing (the
-  // cell belongs to a link workgroup)
-  const bool skip = !LINK && (fl & 2u);
-  int x, y, z;
-  adj_coords(a, c, x, y, z);
-  const uint32_t o = adj_addr(a, x, y, z);
-  R s[Q];
-  uint32_t lmask = 0;  // bit j: direction j is a wall link (q_j >= 0)
-  if constexpr (LINK) {
-    pull_link<S, 6>(a, P, o, x, y, z, qv, s, dsdq);
-    sfor<Q>([&](auto J) {
-      constexpr int j = J;
-      if constexpr (j != 0) lmask |= (qv[j] >= R(0)) ? (1u << j) : 0u;
-    });
-  } else {
-    pull_dense<S>(a, P, o, x, y, z, s);
-  }
   // the inlet's density depends on the pulled values it keeps
   R rho_in = R(0);
   if (x == 0) rho_in = zou_he_in_rho(s, a.uin[0]);
   apply_zou_he(a, s, x);
   const bool post_reset = (a.guard == LBM_GUARD_POST) && guard_bad(s);
   if (post_reset) set_weights(s);
-  R g[Q];
-  sfor<Q>([&](auto I) {
-    constexpr int i = I;
-    g[i] = LM::ld(lam_in, i * N + o, a.lam_inv);
-  });
-  if constexpr (MODE == LBM_MODE_FULL) collide_transpose<R, COLL>(a, s, g, c, fl, skip);
+  load_g(g);
+  if constexpr (MODE == LBM_MODE_FULL) collide_transpose<R, COLL>(a, s, g, gunit, c, fl, skip);
   // momentum-exchange force uses S_{t+1}(x)_opp(j) for links j (forces.py:95-118)
   if constexpr (LINK) {
     if (gF) {
@@ -292,11 +293,48 @@ __device__ __forceinline__ void adjoint_cell(const AdjArgs<typename S::R>& a, co
     zou_he_in_vjp(rho_in, g, a.uin[0], a.uin[1], a.uin[2], gu);
     if (a.u_acc && !skip) {
       const size_t row = size_t(c) / size_t(a.nx);
-      a.u_acc[3 * row] += gu[0];
-      a.u_acc[3 * row + 1] += gu[1];
-      a.u_acc[3 * row + 2] += gu[2];
+      a.u_acc[3 * row] += gu[0] * double(gunit);
+      a.u_acc[3 * row + 1] += gu[1] * double(gunit);
+      a.u_acc[3 * row + 2] += gu[2] * double(gunit);
     }
   }
+}
+
+// One cell: forward recompute + cotangent pull-back + owner writes.
+// MODE FULL: lam_in = dL/dP_{t+1}; MODE STREAM: lam_in = dL/dS_{t+1}.
+template <class S, class LM, int COLL, int MODE, bool LINK>
+__device__ __forceinline__ void adjoint_cell(const AdjArgs<typename S::R>& a, const typename S::T* __restrict__ P,
+                                             const typename LM::T* __restrict__ lam_in,
+                                             typename LM::T* __restrict__ lam_out, const double* __restrict__ gF,
+                                             uint32_t c, int b, uint32_t fl, const typename S::R* qv,
+                                             typename S::R* dsdq) {
+  using R = typename S::R;
+  const size_t N = a.P;
+  // a dense-sweep thread on a link cell computes but writes nothing (the
+  // cell belongs to a link workgroup)
+  const bool skip = !LINK && (fl & 2u);
+  int x, y, z;
+  adj_coords(a, c, x, y, z);
+  const uint32_t o = adj_addr(a, x, y, z);
+  R s[Q];
+  uint32_t lmask = 0;  // bit j: direction j is a wall link (q_j >= 0)
+  if constexpr (LINK) {
+    pull_link<S, 6>(a, P, o, x, y, z, qv, s, dsdq);
+    sfor<Q>([&](auto J) {
+      constexpr int j = J;
+      if constexpr (j != 0) lmask |= (qv[j] >= R(0)) ? (1u << j) : 0u;
+    });
+  } else {
+    pull_dense<S>(a, P, o, x, y, z, s);
+  }
+  R g[Q];
+  auto load_g = [&](R* gg) {
+    sfor<Q>([&](auto I) {
+      constexpr int i = I;
+      gg[i] = LM::ld(lam_in, i * N + o, a.lam_inv);
+    });
+  };
+  cell_transpose<S, COLL, MODE, LINK>(a, s, g, R(1), load_g, c, x, fl, skip, lmask, gF);
   // owner writes and Bouzidi bookkeeping
   if (skip) return;
   LM::st(lam_out, o, g[0], a.lam_scale);
@@ -316,6 +354,69 @@ __device__ __forceinline__ void adjoint_cell(const AdjArgs<typename S::R>& a, co
   });
 }
 
+// Dense cell when every wave's 64 cells share one (y, z) row (nx a multiple
+// of 64, buffers below 4 GiB).  The pull source of direction i and its owner
+// target are the same entry, x - e_i wrapped; its row is wave-uniform, so the
+// state and cotangent accesses go through buffer resources with a scalar
+// byte offset per direction and three per-lane x offsets.  At x = 0 the
+// e_x = +1 pulls and at x = nx - 1 the e_x = -1 pulls read the wrapped
+// neighbour instead of the cell's own value: Zou-He overwrites exactly those
+// populations from the others (boundary.py:113-122, 190-198), so the recomputed
+// step is the same.
+template <class S, class LM, int COLL, int MODE>
+__device__ __forceinline__ void adjoint_row(const AdjArgs<typename S::R>& a, const typename S::T* __restrict__ P,
+                                            const typename LM::T* __restrict__ lam_in,
+                                            typename LM::T* __restrict__ lam_out, uint32_t c) {
+  using R = typename S::R;
+  constexpr uint32_t B = sizeof(typename S::T), BL = sizeof(typename LM::T);
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t cw = __builtin_amdgcn_readfirstlane(c - lane);  // first cell of the wave
+  const uint32_t yz = fdiv(cw, a.dnx);
+  const uint32_t zz = fdiv(yz, a.dny);
+  const int y = int(yz - zz * uint32_t(a.ny)), z = int(zz);
+  const int x = int(cw - yz * uint32_t(a.nx) + lane);
+  const uint32_t xm = uint32_t(x == 0 ? a.nx - 1 : x - 1), xp = uint32_t(x == a.nx - 1 ? 0 : x + 1);
+  const uint32_t fl = a.flags ? uint32_t((a.flags + cw)[lane]) : 0u;
+  const bool skip = fl & 2u;
+  const __amdgpu_buffer_rsrc_t rp = buffer_rsrc(P, a.state_bytes);
+  const __amdgpu_buffer_rsrc_t rli = buffer_rsrc(lam_in, a.lam_bytes);
+  const __amdgpu_buffer_rsrc_t rlo = buffer_rsrc(lam_out, a.lam_bytes);
+  // element offset of the row (y - dy, z - dz), periodic
+  auto row_of = [&](int dy, int dz) -> uint32_t {
+    int ry = y - dy, rz = z - dz;
+    ry = ry < 0 ? ry + a.ny : (ry >= a.ny ? ry - a.ny : ry);
+    rz = rz < 0 ? rz + a.nz : (rz >= a.nz ? rz - a.nz : rz);
+    return (uint32_t(rz) * uint32_t(a.ny) + uint32_t(ry)) * a.RS;
+  };
+  const uint32_t own = yz * a.RS;
+  R s[Q], g[Q];
+  sfor<Q>([&](auto I) {
+    constexpr int i = I;
+    constexpr int ex = EX[i];
+    const uint32_t xs = ex == 1 ? xm : (ex == -1 ? xp : uint32_t(x));
+    const uint32_t e = uint32_t(i) * a.P + row_of(EY[i], EZ[i]);
+    s[i] = S::bld(rp, xs * B, e * B, a.off[i]);
+  });
+  auto load_g = [&](R* gg) {
+    sfor<Q>([&](auto I) {
+      constexpr int i = I;
+      gg[i] = LM::bld(rli, uint32_t(x) * BL, (uint32_t(i) * a.P + own) * BL, R(1));
+    });
+  };
+  // g stays in the stored units (lambda times the storage scale, a power of
+  // two): the transpose commutes with it exactly
+  cell_transpose<S, COLL, MODE, false>(a, s, g, a.lam_inv, load_g, c, x, fl, skip, 0u, nullptr);
+  if (skip) return;
+  sfor<Q>([&](auto I) {
+    constexpr int i = I;
+    constexpr int ex = EX[i];
+    const uint32_t xs = ex == 1 ? xm : (ex == -1 ? xp : uint32_t(x));
+    const uint32_t e = uint32_t(i) * a.P + row_of(EY[i], EZ[i]);
+    const bool normal = !x_fallback<i>(x, a.nx);
+    LM::bst(rlo, xs * BL, e * BL, normal ? g[i] : R(0), R(1));
+  });
+}
+
 // Occupancy target of the dense adjoint sweep: the FP32 MRT transpose fits
 // 128 VGPRs (4 waves per SIMD) with a few spills confined to the inlet plane;
 // the other variants keep the compiler's allocation.
@@ -327,7 +428,7 @@ constexpr int adj_waves_per_eu() {
 // One adjoint step: workgroups [0, nlb) own the link cells (Bouzidi transpose,
 // mu_link, dq), the rest sweep the dense grid; dense threads on link cells
 // compute but write nothing.
-template <class S, class LM, int COLL, int MODE>
+template <class S, class LM, int COLL, int MODE, bool ROW>
 __global__ __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(adj_waves_per_eu<S, COLL>()))) void k_adjoint(
     AdjArgs<typename S::R> a, const typename S::T* __restrict__ P, const typename LM::T* __restrict__ lam_in,
     typename LM::T* __restrict__ lam_out, const double* __restrict__ gF) {
@@ -356,6 +457,10 @@ __global__ __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(adj_w
   const uint32_t tile = bid - uint32_t(a.nlb);
   const uint32_t c = tile * kAdjBlock + threadIdx.x;
   if (c >= a.N) return;
+  if constexpr (ROW) {
+    adjoint_row<S, LM, COLL, MODE>(a, P, lam_in, lam_out, c);
+    return;
+  }
   adjoint_cell<S, LM, COLL, MODE, false>(a, P, lam_in, lam_out, gF, c, -1, a.flags ? uint32_t(a.flags[c]) : 0u,
                                          nullptr, nullptr);
 }
@@ -379,7 +484,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjoint_collide(AdjArgs<typename 
     s[i] = S::ld_c(S0, i * N + o, a.off[i]);
     g[i] = LM::ld(lam_in, i * N + o, a.lam_inv);
   });
-  collide_transpose<R, COLL>(a, s, g, c, a.flags ? uint32_t(a.flags[c]) : 0u, false);
+  collide_transpose<R, COLL>(a, s, g, R(1), c, a.flags ? uint32_t(a.flags[c]) : 0u, false);
   sfor<Q>([&](auto I) {
     constexpr int i = I;
     g_out[i * N + o] = g[i];
@@ -469,6 +574,12 @@ int launch_adjoint(const lbm_step_desc* d, const lbm_adjoint_desc* ad, int mode,
   const int rc = make_adj_args(d, ad, mode, a);
   if (rc) return rc;
   if (!state || !lam_in || !lam_out || lam_in == lam_out) return LBM_E_ARG;
+  // the row-uniform sweep addresses both buffers through 32-bit byte offsets
+  const long long extent = 1LL * (Q - 1) * a.P + (1LL * a.ny * a.nz - 1) * a.RS + a.nx;
+  const long long sb = extent * (long long)sizeof(typename S::T), lb = extent * (long long)sizeof(typename LM::T);
+  const bool row = d->cells_per_thread == 0 && a.nx % 64 == 0 && sb < (1LL << 32) && lb < (1LL << 32);
+  a.state_bytes = row ? uint32_t(sb) : 0u;
+  a.lam_bytes = row ? uint32_t(lb) : 0u;
   const auto* P = static_cast<const typename S::T*>(state);
   const auto* li = static_cast<const typename LM::T*>(lam_in);
   const bool mrt = d->collision == LBM_COLL_MRT;
@@ -481,8 +592,11 @@ int launch_adjoint(const lbm_step_desc* d, const lbm_adjoint_desc* ad, int mode,
   } else {
     auto* lo = static_cast<typename LM::T*>(lam_out);
     const unsigned grid = unsigned(a.nlb + a.ndb);
-#define LBM_ADJ(COLL, MODE) \
-  hipLaunchKernelGGL((k_adjoint<S, LM, COLL, MODE>), grid, kAdjBlock, 0, st, a, P, li, lo, gF)
+#define LBM_ADJ(COLL, MODE)                                                                            \
+  do {                                                                                                 \
+    if (row) hipLaunchKernelGGL((k_adjoint<S, LM, COLL, MODE, true>), grid, kAdjBlock, 0, st, a, P, li, lo, gF);  \
+    else hipLaunchKernelGGL((k_adjoint<S, LM, COLL, MODE, false>), grid, kAdjBlock, 0, st, a, P, li, lo, gF); \
+  } while (0)
     if (mode == LBM_MODE_FULL) {
       if (mrt) LBM_ADJ(LBM_COLL_MRT, LBM_MODE_FULL);
       else LBM_ADJ(LBM_COLL_BGK, LBM_MODE_FULL);

@@ -117,6 +117,16 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x2 __attribute__((ext_vector_type(2)));
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+// Raw buffer access (gfx9 buffer resource, untyped, 32-bit byte offsets): a
+// wave-uniform resource and scalar byte offset (soffset) plus a per-lane byte
+// offset, so the address needs no 64-bit vector arithmetic.  `bytes` bounds
+// the resource; every caller keeps its offsets below it.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, int(bytes), 0x00020000);
+}
+constexpr int kBufNT = 2;  // cache policy of the non-temporal stores
 
 __device__ __forceinline__ float bf16_bits_to_float(unsigned short raw) {
   return __uint_as_float(static_cast<unsigned>(raw) << 16);
@@ -138,6 +148,12 @@ struct StoreF32 {
   __device__ static R ld(const T* p, size_t k, R) { return __builtin_nontemporal_load(p + k); }
   __device__ static R ld_c(const T* p, size_t k, R) { return p[k]; }
   __device__ static void st(T* p, size_t k, R v, R) { __builtin_nontemporal_store(v, p + k); }
+  __device__ static R bld(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, R) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, int(vo), int(so), 0));
+  }
+  __device__ static void bst(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, R v, R) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, int(vo), int(so), kBufNT);
+  }
   template <int V>
   __device__ static void ldv(const T* p, size_t k, R, R* o) {
     if constexpr (V == 4) {
@@ -170,6 +186,12 @@ struct StoreF64 {
   __device__ static R ld(const T* p, size_t k, R) { return __builtin_nontemporal_load(p + k); }
   __device__ static R ld_c(const T* p, size_t k, R) { return p[k]; }
   __device__ static void st(T* p, size_t k, R v, R) { __builtin_nontemporal_store(v, p + k); }
+  __device__ static R bld(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, R) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, int(vo), int(so), 0));
+  }
+  __device__ static void bst(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, R v, R) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, int(vo), int(so), kBufNT);
+  }
   template <int V>
   __device__ static void ldv(const T* p, size_t k, R, R* o) {
 #pragma unroll
@@ -206,6 +228,12 @@ struct StoreHalfDev {
   __device__ static R ld(const T* p, size_t k, R off) { return dec(__builtin_nontemporal_load(bits(p) + k)) + off; }
   __device__ static R ld_c(const T* p, size_t k, R off) { return dec(bits(p)[k]) + off; }
   __device__ static void st(T* p, size_t k, R v, R off) { __builtin_nontemporal_store(enc(v - off), bits(p) + k); }
+  __device__ static R bld(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, R off) {
+    return dec(__builtin_amdgcn_raw_buffer_load_b16(r, int(vo), int(so), 0)) + off;
+  }
+  __device__ static void bst(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, R v, R off) {
+    __builtin_amdgcn_raw_buffer_store_b16(enc(v - off), r, int(vo), int(so), kBufNT);
+  }
   template <int V>
   __device__ static void ldv(const T* p, size_t k, R off, R* o) {
     if constexpr (V == 4) {

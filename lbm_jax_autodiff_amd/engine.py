@@ -100,9 +100,12 @@ class Lattice:
     (examples/10:71-75, reset to eq(1, guard_velocity)).
     ``storage`` f32 / f64 / f16 / bf16 (half formats store f_i - offset_i
     with float32 arithmetic); ``offset`` "weights" (w_i) or "inlet"
-    (f_eq(1, inlet_velocity)) or 19 numbers.  ``cells_per_thread`` (1, 2
-    or 4 when it divides nx; 0 = 1, the fastest) sets how many x-consecutive
-    cells one thread of the dense sweep updates with vector loads.
+    (f_eq(1, inlet_velocity)) or 19 numbers.  ``cells_per_thread`` 0
+    (default, the fastest) runs one cell per thread and, where nx is a
+    multiple of 64, keeps each wave's row addresses in scalar registers
+    (16-bit populations and the adjoint); 1 forces per-lane addresses; 2 or 4
+    (when it divides nx) update that many x-consecutive cells per thread with
+    vector loads.
     ``layout`` places the state buffers of the fused path: "rows" (default)
     interleaves the populations of every x-row, [nz][ny][19][nx], so a row's
     19 populations are one contiguous block; "apart" keeps whole populations

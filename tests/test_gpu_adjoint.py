@@ -25,8 +25,8 @@ def _weights(seed, shape):
 CASES = list(itertools.product(["bgk", "mrt"], ["none", "hard", "soft"], ["none", "post", "pre"], [False, True]))
 
 
-@pytest.mark.parametrize("collision,bounce,guard,smag", CASES)
-def test_adjoint_matches_oracle_autograd_f64(dev, collision, bounce, guard, smag):
+def _check_against_oracle(dev, dims, collision, bounce, guard, smag, cells_per_thread=0):
+    NX, NY, NZ = dims
     steps = 5
     f0 = near_eq_state(NX, NY, NZ, amp=2e-3)
     q = random_q(NX, NY, NZ, seed=4, p_wall=0.1)
@@ -60,7 +60,7 @@ def test_adjoint_matches_oracle_autograd_f64(dev, collision, bounce, guard, smag
     s, p, F = A.rollout(f0d, steps, tau=0.6, q=qd, porosity=phd if bounce == "soft" else None,
                         solid=solid.to(dev) if bounce == "hard" else None, inlet_velocity=ud,
                         collision=collision, smagorinsky_cs=0.1 if smag else None, guard=guard,
-                        guard_velocity=(0.05, 0.0, 0.0), storage="f64")
+                        guard_velocity=(0.05, 0.0, 0.0), storage="f64", cells_per_thread=cells_per_thread)
     loss = (s * wS.to(dev)).sum() + (p * wP.to(dev)).sum() + (F * wF.to(dev)).sum()
     inputs_d = [f0d, qd, ud] + ([phd] if bounce == "soft" else [])
     grads = torch.autograd.grad(loss, inputs_d, allow_unused=True)
@@ -75,6 +75,52 @@ def test_adjoint_matches_oracle_autograd_f64(dev, collision, bounce, guard, smag
         errs[name] = (err, scale, cpu(gd).reshape(-1)[:3].tolist(), gr.reshape(-1)[:3].tolist())
     for name, (err, scale, _, _) in errs.items():
         assert err < 1e-10 * max(1.0, scale), errs
+
+
+@pytest.mark.parametrize("collision,bounce,guard,smag", CASES)
+def test_adjoint_matches_oracle_autograd_f64(dev, collision, bounce, guard, smag):
+    _check_against_oracle(dev, (NX, NY, NZ), collision, bounce, guard, smag)
+
+
+@pytest.mark.parametrize("collision,bounce,guard,smag", CASES)
+def test_adjoint_row_sweep_matches_oracle_autograd_f64(dev, collision, bounce, guard, smag):
+    """nx = 64: every wave's cells share one row and the dense adjoint sweep
+    takes its scalar-addressed path (and reads the wrapped neighbour where
+    Zou-He overwrites the pulled value)."""
+    _check_against_oracle(dev, (64, 6, 4), collision, bounce, guard, smag)
+
+
+# The two sweeps are compiled separately and the compiler contracts a * b + c
+# into fused multiply-adds where it finds them, so their roundings may differ
+# by a few units in the last place; the tolerances allow that and nothing
+# more.  Half storage rounds the states to 16 bits, so a last-place
+# difference of the float32 arithmetic can flip a stored value.
+@pytest.mark.parametrize("storage,rtol", [("f64", 1e-13), ("f32", 2e-6), ("f16", 5e-3)])
+def test_row_sweep_gradient_matches_per_lane_addressing(dev, storage, rtol):
+    """The scalar-addressed sweeps (default where nx % 64 == 0) and the
+    per-lane-addressed ones (cells_per_thread=1) give the same states, forces
+    and gradients (q, porosity, f0, inlet velocity) to rounding."""
+    nx, ny, nz = 64, 20, 12
+    dt = torch.float64 if storage == "f64" else torch.float32
+    por, solid, q = lbm.smooth_sphere_geometry(nx, ny, nz, (-2.0, 0.0, 0.0), 0.5, 10.0, device=dev, out_dtype=dt)
+    u = torch.zeros((3, nx, ny, nz), dtype=dt, device=dev)
+    u[0] = 0.05
+    f0 = lbm.equilibrium(torch.ones((nx, ny, nz), dtype=dt, device=dev), u)
+    out = []
+    for cpt in (0, 1):
+        qq = q.clone().requires_grad_(True)
+        pp = por.clone().requires_grad_(True)
+        ff = f0.clone().requires_grad_(True)
+        uu = torch.tensor([0.05, 0.0, 0.0], dtype=torch.float64, device=dev, requires_grad=True)
+        S, P, F = A.rollout(ff, 13, tau=0.55, q=qq, porosity=pp, inlet_velocity=uu, collision="mrt", guard="pre",
+                            smagorinsky_cs=0.1, storage=storage, offset="inlet", checkpoint=4, cells_per_thread=cpt)
+        loss = F[:, 0].sum() + S.double().sum() * 1e-3
+        g = torch.autograd.grad(loss, (qq, pp, ff, uu))
+        out.append([cpu(t).double() for t in (S, P, F) + g])
+    for name, a, b in zip(["S", "P", "F", "dq", "dphi", "df0", "du"], *out):
+        scale = float(a.abs().max())
+        assert scale > 0.0, name
+        assert float((a - b).abs().max()) <= rtol * scale, name
 
 
 @pytest.mark.parametrize("guard", ["none", "pre"])

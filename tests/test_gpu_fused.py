@@ -169,6 +169,38 @@ def test_state_layout_does_not_change_results(dev, storage, vec):
             assert torch.equal(a, b)
 
 
+ROW_CASES = list(itertools.product(["f16", "bf16"], ["bgk", "mrt"], ["none", "hard", "soft"],
+                                   ["none", "post", "pre"], [True, False]))
+
+
+@pytest.mark.parametrize("storage,collision,bounce,guard,periodic", ROW_CASES)
+def test_row_sweep_matches_per_lane_addressing(dev, storage, collision, bounce, guard, periodic):
+    """At nx = 64 the 16-bit sweep keeps each wave's row addresses in scalar
+    registers (and reads the wrapped neighbour where Zou-He overwrites the
+    pulled value); cells_per_thread=1 forces per-lane addresses.  The two
+    kernels are compiled separately (fused multiply-adds may round
+    differently), so states and forces agree to one 16-bit rounding of a
+    deviation, with Smagorinsky, links and clamped y/z faces."""
+    nx, ny, nz = 64, 7, 5
+    steps = 6
+    f0 = near_eq_state(nx, ny, nz, amp=2e-3)
+    q = random_q(nx, ny, nz, seed=4, p_wall=0.1)
+    g = torch.Generator().manual_seed(2)
+    solid = (torch.rand((nx, ny, nz), generator=g) < 0.1).to(torch.int32)
+    phi = 0.3 * torch.rand((nx, ny, nz), generator=g, dtype=torch.float32)
+    outs = []
+    for cpt in (0, 1):
+        lat = Lattice(nx, ny, nz, tau=0.6, inlet_velocity=(0.05, 0.0, 0.0), collision=collision,
+                      smagorinsky_cs=0.1, solid=solid.to(dev) if bounce == "hard" else None,
+                      porosity=phi.to(dev) if bounce == "soft" else None, q=q.to(dev), guard=guard,
+                      guard_velocity=(0.05, 0.0, 0.0), periodic_yz=periodic, storage=storage, device=dev,
+                      offset="inlet", cells_per_thread=cpt)
+        outs.append([cpu(t).double() for t in lat.run(f0.to(torch.float32).to(dev), steps)])
+    tol = 5e-5 if storage == "f16" else 5e-4
+    for a, b in zip(*outs):
+        assert float((a - b).abs().max()) <= tol * max(1.0, float(a.abs().max()))
+
+
 def test_default_state_layouts(dev):
     lat = Lattice(256, 128, 128, tau=0.6, device=dev)
     assert lat.layout == "rows" and lat.P == 256 and lat.RS == 19 * 256
