@@ -132,11 +132,8 @@ __device__ __forceinline__ float bf16_bits_to_float(unsigned short raw) {
   return __uint_as_float(static_cast<unsigned>(raw) << 16);
 }
 __device__ __forceinline__ unsigned short float_to_bf16_bits(float d) {
-  // round to nearest even; NaN stays NaN
-  unsigned u = __float_as_uint(d);
-  if ((u & 0x7f800000u) == 0x7f800000u) return static_cast<unsigned short>((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return static_cast<unsigned short>(u >> 16);
+  // round to nearest even, NaN stays NaN: one v_cvt_pk_bf16_f32 on gfx950
+  return __builtin_bit_cast(unsigned short, static_cast<__bf16>(d));
 }
 __device__ __forceinline__ float f16_bits_to_float(unsigned short raw) { return __half2float(__ushort_as_half(raw)); }
 __device__ __forceinline__ unsigned short float_to_f16_bits(float d) { return __half_as_ushort(__float2half_rn(d)); }

@@ -201,6 +201,33 @@ def test_row_sweep_matches_per_lane_addressing(dev, storage, collision, bounce, 
         assert float((a - b).abs().max()) <= tol * max(1.0, float(a.abs().max()))
 
 
+@pytest.mark.parametrize("storage,tdt", [("f16", torch.float16), ("bf16", torch.bfloat16)])
+def test_half_storage_rounds_to_nearest_even(dev, storage, tdt):
+    """Encoding a state rounds to nearest, ties to even, exactly as torch's
+    conversion does: exact ties, values just above and below them, and
+    random values over a wide range (offsets 0, so the stored value is f)."""
+    nx, ny, nz = 64, 4, 4
+    n = 19 * nx * ny * nz
+    g = torch.Generator().manual_seed(7)
+    if storage == "bf16":
+        hi = torch.randint(0, 0x7f00, (n,), generator=g, dtype=torch.int64)  # finite magnitudes
+        low = torch.tensor([0x8000, 0x8001, 0x7fff, 0x0000], dtype=torch.int64)[torch.arange(n) % 4]
+        bits = (hi << 16) | low
+        sign = torch.randint(0, 2, (n,), generator=g, dtype=torch.int64) << 31
+        f = torch.tensor(((bits | sign) & 0xffffffff).to(torch.int32).numpy().view("float32"))
+    else:
+        f = torch.randn(n, generator=g, dtype=torch.float64) * torch.exp2(torch.randint(-20, 12, (n,), generator=g))
+        f = f.to(torch.float32)
+        # exact ties: midpoints of neighbouring half values
+        h = torch.randn(n // 4, generator=g).to(torch.float16)
+        up = torch.nextafter(h, torch.full_like(h, float("inf")))
+        f[: n // 4] = ((h.to(torch.float64) + up.to(torch.float64)) / 2).to(torch.float32)
+    f = f.reshape(19, nx, ny, nz)
+    lat = Lattice(nx, ny, nz, tau=0.6, storage=storage, offset=[0.0] * 19, device=dev)
+    back = cpu(lat.from_state(lat.to_state(f.to(dev))))
+    assert torch.equal(back, f.to(tdt).to(torch.float32))
+
+
 def test_default_state_layouts(dev):
     lat = Lattice(256, 128, 128, tau=0.6, device=dev)
     assert lat.layout == "rows" and lat.P == 256 and lat.RS == 19 * 256
