@@ -18,7 +18,10 @@
 // (link_cell_update): a link path that needs more either lowers the occupancy
 // of every workgroup or, held to the budget, spills and finishes last under
 // the full memory load; as a separate launch ahead of the sweep it cost
-// 15 us plus a launch gap per step at 256 x 128 x 128.
+// 15 us plus a launch gap per step at 256 x 128 x 128.  With 16-bit
+// populations the sweep is bound by vector instructions rather than memory,
+// and where nx is a multiple of 64 it takes each wave's row addresses from
+// scalar registers (dense_row).
 #include "lbm_physics.hpp"
 
 namespace lbm {
