@@ -220,8 +220,8 @@ def main():
                     help="also run a plain 1 GiB device copy (known bytes) for hardware-counter scaling")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build targets one GPU; multi-GPU runs are not part of it "
-              "(x-slab sharding is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py: one GPU only; the x-slab sharding over several GPUs (a depth-1 halo of the "
+              "five e_x = +-1 populations per face and step) is designed but not built", file=sys.stderr)
         sys.exit(2)
     if not torch.cuda.is_available():
         print("bench.py: no HIP device", file=sys.stderr)
