@@ -12,7 +12,8 @@ cell update for float32 storage, 76 B for half storage) from HIP events
 around each launch, half-storage forward MLUPS, forward+adjoint MLUPS
 (store-all), the dCd/dr pipeline of the 256x128x128 precision study
 (checkpoints every 100 steps) in float32, float16 and float64 storage with
-its wall-time ratio and errors against float64, and the CPU oracle (torch,
+its wall-time ratio and errors against float64, the forward rate of the
+small configurations (launch-latency bound), and the CPU oracle (torch,
 float32) timed on the host over a bounded sample.
 
 Each timed loop follows W untimed warm-up steps and further untimed steps
@@ -134,6 +135,32 @@ def time_forward_adjoint(storage, steps, device, checkpoint="all"):
     e1.record()
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) * 1e-3, float(g), float(loss.detach())
+
+
+def small_grids(device, steps=2000):
+    """Forward MLUPS of the small configurations (64x21x21 BGK channel,
+    64x32x32 and 96x48x48 MRT spheres) with per-step forces.  Their states fit
+    the 256 MB last-level cache and a step is a few microseconds, so these
+    rates are set by the per-launch latency, not by HBM."""
+    from lbm_jax_autodiff_amd import protocols as Pr
+    from lbm_jax_autodiff_amd.geometry import compute_q_from_solid
+    from lbm_jax_autodiff_amd.engine import Lattice
+    out = {}
+    nx, ny, nz = 64, 21, 21
+    solid = torch.zeros((nx, ny, nz), dtype=torch.int32, device=device)
+    solid[:, 0, :] = 1
+    solid[:, ny - 1, :] = 1
+    chan = Lattice(nx, ny, nz, tau=0.8, inlet_velocity=(0.05, 0.0, 0.0), collision="bgk", solid=solid,
+                   q=compute_q_from_solid(solid, nx, ny, nz, 0.5), device=device)
+    cases = (("channel_64x21x21_bgk", chan),
+             ("sphere_64x32x32_mrt", Pr.sphere_lattice(64, 32, 32, (-2.0, 0.0, 0.0), guard="pre")[0]),
+             ("sphere_96x48x48_mrt", Pr.sphere_lattice(96, 48, 48, (-2.0, 0.0, 0.0), guard="pre")[0]))
+    for name, lat in cases:
+        loop_ms, _, _ = time_forward(lat, initial_state(lat, device), steps, 200)
+        out[name] = {"us_per_step": round(loop_ms * 1e3, 2),
+                     "mlups": round(lat.nx * lat.ny * lat.nz / (loop_ms * 1e-3) / 1e6, 1)}
+        log(f"small grid {name}: {loop_ms * 1e3:.2f} us/step")
+    return out
 
 
 def dcd_dr_study(steps, device):
@@ -280,6 +307,7 @@ def main():
         # forward 152 / 76 B plus adjoint: P_t, lambda in, lambda out (float32
         # cotangents for f32 storage, scaled half cotangents for f16)
         extras["fwd_adj_bytes_per_cell_update"] = {"f32": 380.0, "f16": 190.0}
+        extras["small_grids_f32_fwd"] = small_grids(dev)
         if args.study_steps > 0:
             extras["dcd_dr_study"] = dcd_dr_study(args.study_steps, dev)
 
