@@ -55,6 +55,8 @@ def scatter_plan(lat: Lattice):
       q <  1/2: S_i(x) = 2q P_j(x) + (1 - 2q) P_j(x + e_i)   (x + e_i wrapped)
     and the force term F += e_j P_j(x) (forces.py:95-118).  Entries are
     ordered by target and then by source, so the sums are deterministic.
+    Built with tensor operations on the links' device (one host sync, for the
+    number of wall links).
     """
     n = lat.links.n
     if n == 0:
@@ -62,55 +64,45 @@ def scatter_plan(lat: Lattice):
     nx, ny, nz = lat.nx, lat.ny, lat.nz
     N = lat.P  # population stride of the cotangent buffers
     RS = lat.RS  # row stride
-    cell = lat.links.cell.detach().to("cpu").numpy().astype(np.int64)
-    q = lat.links.q.detach().to("cpu").numpy()  # (18, n)
+    dev = lat.links.cell.device
+    cell = lat.links.cell.to(torch.int64)
+    q = lat.links.q.to(torch.float64)  # (18, n)
     x = cell % nx
     y = (cell // nx) % ny
     z = cell // (nx * ny)
     off = (z * ny + y) * RS + x  # element offset of each link cell in a population
-    keys, srcs, coefs = [], [], []
-    b = np.arange(n, dtype=np.int64)
-    for j in range(1, 19):
-        qj = q[j - 1]
-        has = qj >= 0.0
-        if not has.any():
-            continue
-        i = _OPP[j]
-        bb, qq, cc = b[has], qj[has], off[has]
-        src = i * n + bb
-        hi = qq >= 0.5
-        inv2q = 1.0 / (2.0 * np.clip(qq, 0.5, 1.0))
-        twoq = 2.0 * np.clip(qq, 0.0, 0.5)
-        # own P_j(x): both branches
-        keys.append(j * N + cc)
-        srcs.append(src)
-        coefs.append(np.where(hi, inv2q, twoq))
-        # own P_i(x): high branch
-        keys.append(i * N + cc[hi])
-        srcs.append(src[hi])
-        coefs.append(1.0 - inv2q[hi])
-        # P_j(x + e_i): low branch
-        lo = ~hi
-        fx = (x[has][lo] + _E[i, 0]) % nx
-        fy = (y[has][lo] + _E[i, 1]) % ny
-        fz = (z[has][lo] + _E[i, 2]) % nz
-        keys.append(j * N + (fz * ny + fy) * RS + fx)
-        srcs.append(src[lo])
-        coefs.append(1.0 - twoq[lo])
-        # force term on P_j(x)
-        keys.append(j * N + cc)
-        srcs.append(np.full(cc.shape, -1 - j, dtype=np.int64))
-        coefs.append(np.zeros(cc.shape))
-    key = np.concatenate(keys)
-    src = np.concatenate(srcs)
-    coef = np.concatenate(coefs)
-    order = np.lexsort((src, key))
-    key, src, coef = key[order], src[order], coef[order]
-    uniq, start = np.unique(key, return_index=True)
-    start = np.concatenate([start, [key.size]]).astype(np.int32)
-    dev = lat.device
-    return (torch.from_numpy(uniq.astype(np.int32)).to(dev), torch.from_numpy(start).to(dev),
-            torch.from_numpy(src.astype(np.int32)).to(dev), torch.from_numpy(coef).to(dev))
+    jm1, b = torch.nonzero(q >= 0.0, as_tuple=True)  # every wall link (j - 1, link index)
+    j = jm1 + 1
+    i = torch.as_tensor(_OPP, device=dev)[j]
+    e = torch.as_tensor(_E, device=dev)
+    qq = q[jm1, b]
+    cc = off[b]
+    src = i * n + b
+    hi = qq >= 0.5
+    inv2q = 1.0 / (2.0 * qq.clamp(0.5, 1.0))
+    twoq = 2.0 * qq.clamp(0.0, 0.5)
+    # P_j(x + e_i), the low branch's second neighbour, all axes wrapped
+    fx = (x[b] + e[i, 0]) % nx
+    fy = (y[b] + e[i, 1]) % ny
+    fz = (z[b] + e[i, 2]) % nz
+    key = torch.cat([
+        j * N + cc,                                                    # own P_j(x): both branches
+        torch.where(hi, i * N + cc, j * N + (fz * ny + fy) * RS + fx),  # own P_i(x) | P_j(x + e_i)
+        j * N + cc,                                                    # force term on P_j(x)
+    ])
+    srcs = torch.cat([src, src, -1 - j])
+    coef = torch.cat([torch.where(hi, inv2q, twoq), torch.where(hi, 1.0 - inv2q, 1.0 - twoq),
+                      torch.zeros_like(qq)])
+    # order by target, then by source ((target, source) pairs are unique)
+    o1 = torch.argsort(srcs, stable=True)
+    o2 = torch.argsort(key[o1], stable=True)
+    order = o1[o2]
+    key, srcs, coef = key[order], srcs[order], coef[order]
+    uniq, counts = torch.unique_consecutive(key, return_counts=True)
+    start = torch.cat([torch.zeros(1, dtype=torch.int64, device=dev), torch.cumsum(counts, 0)])
+    out = lat.device
+    return (uniq.to(torch.int32).to(out), start.to(torch.int32).to(out), srcs.to(torch.int32).to(out),
+            coef.to(out))
 
 
 class _Checkpoints:
@@ -192,6 +184,9 @@ class _Checkpoints:
 class _RolloutFn(torch.autograd.Function):
     @staticmethod
     def forward(ctx, f0, q, porosity, u_in, cfg):
+        # outputs the loss does not use arrive as None instead of zero tensors
+        # (no full-grid seed reads or adds for them)
+        ctx.set_materialize_grads(False)
         n_steps = cfg["n_steps"]
         vel = [float(v) for v in u_in.detach().reshape(-1).to("cpu", torch.float64).tolist()]
         lat = Lattice(cfg["nx"], cfg["ny"], cfg["nz"], tau=cfg["tau"], inlet_velocity=vel, rho_out=cfg["rho_out"],
@@ -336,10 +331,10 @@ class _RolloutFn(torch.autograd.Function):
                 g_f0 = F.pop_view(g_s0).to(ctx.f0_dtype).clone()
         g_q = None
         if want_q:
-            dense = torch.zeros((19, N), dtype=torch.float64, device=dev)
+            dense = torch.zeros((19, N), dtype=ctx.q_dtype, device=dev)
             cells = lat.links.cell.long()
-            dense[1:, :].index_copy_(1, cells, dq)
-            g_q = F.pop_view(dense.reshape(19, nz, ny, nx)).to(ctx.q_dtype)
+            dense[1:, :].index_copy_(1, cells, dq.to(ctx.q_dtype))
+            g_q = F.pop_view(dense.reshape(19, nz, ny, nx))
         g_phi = F.scalar_view(phi_acc.reshape(nz, ny, nx)).to(ctx.phi_dtype) if want_phi else None
         g_u = u_acc.sum(0).to(ctx.u_dtype) if want_u else None
         return g_f0, g_q, g_phi, g_u, None
