@@ -95,6 +95,14 @@ def test_adjoint_row_sweep_matches_oracle_autograd_f64(dev, collision, bounce, g
 # by a few units in the last place; the tolerances allow that and nothing
 # more.  Half storage rounds the states to 16 bits, so a last-place
 # difference of the float32 arithmetic can flip a stored value.
+@pytest.mark.parametrize("collision,bounce,guard", [("mrt", "hard", "pre"), ("bgk", "soft", "post"),
+                                                   ("mrt", "soft", "none")])
+def test_adjoint_row_sweep_two_waves_per_row_matches_oracle(dev, collision, bounce, guard):
+    """nx = 128: two waves per row and two rows per workgroup in the
+    scalar-addressed adjoint sweep."""
+    _check_against_oracle(dev, (128, 4, 3), collision, bounce, guard, True)
+
+
 @pytest.mark.parametrize("storage,rtol", [("f64", 1e-13), ("f32", 2e-6), ("f16", 5e-3)])
 def test_row_sweep_gradient_matches_per_lane_addressing(dev, storage, rtol):
     """The scalar-addressed sweeps (default where nx % 64 == 0) and the

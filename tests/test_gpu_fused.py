@@ -173,15 +173,16 @@ ROW_CASES = list(itertools.product(["f16", "bf16"], ["bgk", "mrt"], ["none", "ha
                                    ["none", "post", "pre"], [True, False]))
 
 
+@pytest.mark.parametrize("dims", [(64, 7, 5), (128, 5, 3)])
 @pytest.mark.parametrize("storage,collision,bounce,guard,periodic", ROW_CASES)
-def test_row_sweep_matches_per_lane_addressing(dev, storage, collision, bounce, guard, periodic):
+def test_row_sweep_matches_per_lane_addressing(dev, storage, collision, bounce, guard, periodic, dims):
     """At nx = 64 the 16-bit sweep keeps each wave's row addresses in scalar
     registers (and reads the wrapped neighbour where Zou-He overwrites the
     pulled value); cells_per_thread=1 forces per-lane addresses.  The two
     kernels are compiled separately (fused multiply-adds may round
     differently), so states and forces agree to one 16-bit rounding of a
     deviation, with Smagorinsky, links and clamped y/z faces."""
-    nx, ny, nz = 64, 7, 5
+    nx, ny, nz = dims  # nx = 128: two waves per row, two rows per workgroup
     steps = 6
     f0 = near_eq_state(nx, ny, nz, amp=2e-3)
     q = random_q(nx, ny, nz, seed=4, p_wall=0.1)
