@@ -27,6 +27,7 @@
 namespace lbm {
 
 constexpr int kStepBlock = 256;
+constexpr int kStepBlockLarge = 512;
 
 template <typename R>
 struct StepArgs {
@@ -441,8 +442,8 @@ __device__ __forceinline__ void link_cell_update(const StepArgs<typename S::R>& 
   store_cell<S>(a, dst, o, s);
 }
 
-template <class S, int COLL, int MODE, int V>
-__global__ __launch_bounds__(kStepBlock) __attribute__((amdgpu_waves_per_eu(step_waves_per_eu<S, V>()))) void k_step(
+template <class S, int COLL, int MODE, int V, int BS = kStepBlock>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(step_waves_per_eu<S, V>()))) void k_step(
     StepArgs<typename S::R> a, const typename S::T* __restrict__ src, typename S::T* __restrict__ dst,
     double* __restrict__ fpart) {
   using R = typename S::R;
@@ -451,12 +452,12 @@ __global__ __launch_bounds__(kStepBlock) __attribute__((amdgpu_waves_per_eu(step
 
   if (MODE != LBM_MODE_COLLIDE && int(bid) < a.nlb) {
     // ---------------- link cells ----------------
-    __shared__ double fl[kStepBlock][3];
-    const int b = int(bid) * kStepBlock + int(threadIdx.x);
+    __shared__ double fl[BS][3];
+    const int b = int(bid) * BS + int(threadIdx.x);
     fl[threadIdx.x][0] = fl[threadIdx.x][1] = fl[threadIdx.x][2] = 0.0;
     if (b < a.n_links) link_cell_update<S, COLL, MODE, 6>(a, src, dst, b, fl[threadIdx.x]);
     if (fpart) {
-      __shared__ double red[3][kStepBlock / 64];
+      __shared__ double red[3][BS / 64];
       const double fx = wave_sum(fl[threadIdx.x][0]);
       const double fy = wave_sum(fl[threadIdx.x][1]);
       const double fz = wave_sum(fl[threadIdx.x][2]);
@@ -470,7 +471,7 @@ __global__ __launch_bounds__(kStepBlock) __attribute__((amdgpu_waves_per_eu(step
       if (threadIdx.x < 3) {
         double acc = 0.0;
 #pragma unroll
-        for (int k = 0; k < kStepBlock / 64; ++k) acc += red[threadIdx.x][k];
+        for (int k = 0; k < BS / 64; ++k) acc += red[threadIdx.x][k];
         fpart[size_t(bid) * 3 + threadIdx.x] = acc;
       }
     }
@@ -481,10 +482,10 @@ __global__ __launch_bounds__(kStepBlock) __attribute__((amdgpu_waves_per_eu(step
   const uint32_t first = (MODE == LBM_MODE_COLLIDE) ? 0u : uint32_t(a.nlb);
   const uint32_t tile = bid - first;
   if constexpr (V > 1) {
-    dense_vec<S, COLL, MODE, V>(a, src, dst, tile * kStepBlock + threadIdx.x);
+    dense_vec<S, COLL, MODE, V>(a, src, dst, tile * BS + threadIdx.x);
     return;
   }
-  const uint32_t c = tile * kStepBlock + threadIdx.x;
+  const uint32_t c = tile * BS + threadIdx.x;
   if (c >= a.N) return;
   if constexpr (V == 0) {
     if (a.periodic) dense_row<S, COLL, MODE, true>(a, src, dst, c);
@@ -603,14 +604,31 @@ inline int choose_vec(const lbm_step_desc* d, uint32_t pstride, uint32_t rstride
   return 1;
 }
 
+// Threads per workgroup of a step launch: 512 for the one-cell-per-thread
+// float32 sweep of grids of 2^21 cells or more (1% faster than 256 at
+// 256 x 128 x 128; 768 and 1024 are slower), 256 otherwise (the 16-bit sweep
+// is 7% slower at 512, and small grids lose parallelism: 96 x 48 x 48 went
+// from 14.4 to 18.4 us per step).  Also sets the number of force slots.
+inline int step_block(const lbm_step_desc* d) {
+  long long ps = 0, rs = 0;
+  if (!state_layout(d, ps, rs)) return kStepBlock;
+  const long long cells = 1LL * d->nx * d->ny * d->nz;
+  const bool one_cell = choose_vec(d, uint32_t(ps), uint32_t(rs)) == 1;
+  return (d->storage == LBM_F32 && one_cell && cells >= (1LL << 21)) ? kStepBlockLarge : kStepBlock;
+}
+
 template <class S, int COLL, int MODE>
-int launch_step_cfg(const StepArgs<typename S::R>& a, int V, const typename S::T* in, typename S::T* out,
+int launch_step_cfg(const StepArgs<typename S::R>& a, int V, int bs, const typename S::T* in, typename S::T* out,
                     double* fpart, hipStream_t s) {
   const unsigned grid = unsigned(a.nlb + a.ndb);
+  constexpr bool f32 = std::is_same<S, StoreF32>::value;
   if (V == 4) hipLaunchKernelGGL((k_step<S, COLL, MODE, 4>), grid, kStepBlock, 0, s, a, in, out, fpart);
   else if (V == 2) hipLaunchKernelGGL((k_step<S, COLL, MODE, 2>), grid, kStepBlock, 0, s, a, in, out, fpart);
   else if (MODE != LBM_MODE_COLLIDE && a.row_waves)
     hipLaunchKernelGGL((k_step<S, COLL, MODE, 0>), grid, kStepBlock, 0, s, a, in, out, fpart);
+  else if (f32 && bs == kStepBlockLarge)
+    hipLaunchKernelGGL((k_step<S, COLL, MODE, 1, f32 ? kStepBlockLarge : kStepBlock>), grid, kStepBlockLarge, 0, s,
+                       a, in, out, fpart);
   else hipLaunchKernelGGL((k_step<S, COLL, MODE, 1>), grid, kStepBlock, 0, s, a, in, out, fpart);
   return hipGetLastError() == hipSuccess ? LBM_OK : LBM_E_LAUNCH;
 }
@@ -634,21 +652,23 @@ int launch_step(const lbm_step_desc* d, int mode, const void* src, void* dst, do
                    bytes < (1LL << 32)) ? 1 : 0;
     a.state_bytes = a.row_waves ? uint32_t(bytes) : 0u;
   }
-  const long long cells_per_block = 1LL * kStepBlock * V;
+  const int bs = step_block(d);
+  a.nlb = (mode == LBM_MODE_COLLIDE) ? 0 : (a.n_links + bs - 1) / bs;
+  const long long cells_per_block = 1LL * bs * V;
   a.ndb = int((1LL * a.N + cells_per_block - 1) / cells_per_block);
   const auto* in = static_cast<const typename S::T*>(src);
   auto* out = static_cast<typename S::T*>(dst);
   const bool mrt = a.coll == LBM_COLL_MRT;
   switch (mode) {
     case LBM_MODE_FULL:
-      return mrt ? launch_step_cfg<S, LBM_COLL_MRT, LBM_MODE_FULL>(a, V, in, out, fpart, s)
-                 : launch_step_cfg<S, LBM_COLL_BGK, LBM_MODE_FULL>(a, V, in, out, fpart, s);
+      return mrt ? launch_step_cfg<S, LBM_COLL_MRT, LBM_MODE_FULL>(a, V, bs, in, out, fpart, s)
+                 : launch_step_cfg<S, LBM_COLL_BGK, LBM_MODE_FULL>(a, V, bs, in, out, fpart, s);
     case LBM_MODE_COLLIDE:
-      return mrt ? launch_step_cfg<S, LBM_COLL_MRT, LBM_MODE_COLLIDE>(a, V, in, out, fpart, s)
-                 : launch_step_cfg<S, LBM_COLL_BGK, LBM_MODE_COLLIDE>(a, V, in, out, fpart, s);
+      return mrt ? launch_step_cfg<S, LBM_COLL_MRT, LBM_MODE_COLLIDE>(a, V, bs, in, out, fpart, s)
+                 : launch_step_cfg<S, LBM_COLL_BGK, LBM_MODE_COLLIDE>(a, V, bs, in, out, fpart, s);
     case LBM_MODE_STREAM:
-      return mrt ? launch_step_cfg<S, LBM_COLL_MRT, LBM_MODE_STREAM>(a, V, in, out, fpart, s)
-                 : launch_step_cfg<S, LBM_COLL_BGK, LBM_MODE_STREAM>(a, V, in, out, fpart, s);
+      return mrt ? launch_step_cfg<S, LBM_COLL_MRT, LBM_MODE_STREAM>(a, V, bs, in, out, fpart, s)
+                 : launch_step_cfg<S, LBM_COLL_BGK, LBM_MODE_STREAM>(a, V, bs, in, out, fpart, s);
   }
   return LBM_E_ARG;
 }
@@ -672,7 +692,8 @@ int64_t lbm_state_stride(const lbm_step_desc* d) {
 
 int lbm_step_force_slots(const lbm_step_desc* d) {
   if (!d || d->n_links < 0) return 0;
-  return (d->n_links + kStepBlock - 1) / kStepBlock;
+  const int bs = step_block(d);
+  return (d->n_links + bs - 1) / bs;
 }
 
 int lbm_step(const lbm_step_desc* d, int mode, const void* src, void* dst, double* force_partial,
