@@ -38,7 +38,7 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE / WRITE_SIZE counter passes scaled by a 1 GiB copy
 # (profiles/k_step_mrt_f32_256x128x128_hbm_traffic.md); re-measure when the
 # kernel changes
-MEASURED_TRAFFIC_BYTES = 658.8e6
+MEASURED_TRAFFIC_BYTES = 654.5e6
 NX, NY, NZ = 256, 128, 128
 
 
