@@ -189,6 +189,32 @@ def dcd_dr_study(steps, device):
     return out
 
 
+def c3_gradient(device, steps=2000):
+    """dCd/dr of the 96x48x48 sphere (r = 6 lattice units, centre X = (24, 24, 24),
+    i.e. world (-2, 0, 0); MRT + Bouzidi q(r), tau 0.518, loss = mean Cd of the
+    last 600 steps, every state kept in HBM) in float32 and float64 storage.
+    The first call allocates the stored states (33.6 GB in float32); the
+    second call of the same size reuses them from torch's caching allocator
+    and times the pipeline itself."""
+    from lbm_jax_autodiff_amd.protocols import sphere_loss_and_grad, world_center
+    nx, ny, nz = 96, 48, 48
+    c = world_center(nx, ny, nz, (24, 24, 24))
+    out = {"steps": steps, "checkpoint": "all"}
+    torch.cuda.empty_cache()
+    for st in ("f32", "f64"):
+        first = sphere_loss_and_grad(nx, ny, nz, steps, c, 0.5, storage=st)
+        r = sphere_loss_and_grad(nx, ny, nz, steps, c, 0.5, storage=st)
+        out[st] = {"first_call_s": round(first["fwd_adj_s"], 4), "s": round(r["fwd_adj_s"], 4),
+                   "fwd_s": round(r["fwd_s"], 4), "fwd_adj_mlups": round(r["fwd_adj_mlups"], 1),
+                   "cd": r["loss"], "dcd_dr": r["dloss_dr_world"]}
+        torch.cuda.empty_cache()
+        log(f"96x48x48 dCd/dr {st}: first call {first['fwd_adj_s']:.3f} s, second {r['fwd_adj_s']:.3f} s "
+            f"(forward {r['fwd_s']:.3f} s), dCd/dr={r['dloss_dr_world']:.9f}")
+    ref = out["f64"]["dcd_dr"]
+    out["f32"]["dcd_dr_rel_err"] = abs(out["f32"]["dcd_dr"] - ref) / abs(ref)
+    return out
+
+
 def cpu_baseline(sample_steps=2):
     """Oracle (torch CPU float32, the reference's vectorised op structure) on
     the same workload, timed on the host cores over a few steps."""
@@ -242,6 +268,7 @@ def main():
     ap.add_argument("--adj-steps", type=int, default=200)
     ap.add_argument("--study-steps", type=int, default=28000,
                     help="steps of the dCd/dr precision study (0 skips it)")
+    ap.add_argument("--skip-c3", action="store_true", help="skip the 96x48x48 store-all dCd/dr timing")
     ap.add_argument("--no-extras", action="store_true", help="forward float32 only")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (known bytes) for hardware-counter scaling")
@@ -308,6 +335,8 @@ def main():
         # cotangents for f32 storage, scaled half cotangents for f16)
         extras["fwd_adj_bytes_per_cell_update"] = {"f32": 380.0, "f16": 190.0}
         extras["small_grids_f32_fwd"] = small_grids(dev)
+        if not args.skip_c3:
+            extras["c3_dcd_dr"] = c3_gradient(dev)
         if args.study_steps > 0:
             extras["dcd_dr_study"] = dcd_dr_study(args.study_steps, dev)
 
