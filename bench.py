@@ -12,9 +12,10 @@ cell update for float32 storage, 76 B for half storage) from HIP events
 around each launch, half-storage forward MLUPS, forward+adjoint MLUPS
 (store-all), the dCd/dr pipeline of the 256x128x128 precision study
 (checkpoints every 100 steps) in float32, float16 and float64 storage with
-its wall-time ratio and errors against float64, the forward rate of the
-small configurations (launch-latency bound), and the CPU oracle (torch,
-float32) timed on the host over a bounded sample.
+its wall-time ratio and errors against float64, the 96x48x48 store-all
+dCd/dr (first and second call, float32 against float64), the forward rate of
+the small configurations (launch-latency bound), and the CPU oracle (torch,
+float32) timed on the host over a bounded sample (about 20 s).
 
 Each timed loop follows W untimed warm-up steps and further untimed steps
 until 0.25 s have passed (fresh buffers run up to 10% slower for the first
@@ -264,7 +265,7 @@ def main():
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=40)
     ap.add_argument("--skip-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=16)
+    ap.add_argument("--cpu-steps", type=int, default=32)
     ap.add_argument("--adj-steps", type=int, default=200)
     ap.add_argument("--study-steps", type=int, default=28000,
                     help="steps of the dCd/dr precision study (0 skips it)")
