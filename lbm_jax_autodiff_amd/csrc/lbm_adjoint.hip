@@ -496,7 +496,10 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjoint_collide(AdjArgs<typename 
 // (+ gF . e_j for force terms, encoded src = -1 - j), summed in list order.
 // The target's old value and a batch of SCATTER_BATCH entries are loaded
 // before any of them is used, so a target costs about three dependent round
-// trips instead of two per entry.
+// trips instead of two per entry. Entry (x, j) receives at most four additions
+// (the own term of link (x, opp j), the high-branch and force terms of link
+// (x, j), the low-branch neighbour term of link (x + e_j, opp j)), so one batch
+// covers every target; longer lists would take more batches.
 constexpr int SCATTER_BATCH = 4;
 
 template <typename R, class LM>

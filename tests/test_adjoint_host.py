@@ -196,6 +196,11 @@ def test_scatter_plan_is_sorted_and_complete():
     n_hi = int((qs[1:, cells][qs[1:, cells] >= 0] >= 0.5).sum())
     assert int(src.numel()) == n_force + n_links + n_links  # own + (own i | neighbour) + force
     assert n_hi <= n_links
+    # entry (x, j) gets at most four additions: the own term of link (x, opp j), the high-branch
+    # and force terms of link (x, j), the low-branch neighbour term of link (x + e_j, opp j);
+    # one load batch of the scatter kernel covers it
+    per_target = (start[1:] - start[:-1]).numpy()
+    assert per_target.min() >= 1 and per_target.max() <= 4
 
 
 def _plan_by_entry(plan, decode):
