@@ -29,6 +29,29 @@ def require_cuda(t: torch.Tensor, name: str) -> None:
                          "this package has no CPU path")
 
 
+def require_shape(t: torch.Tensor, name: str, shape, device=None) -> None:
+    """t must be a device tensor of exactly ``shape`` (and on ``device`` when
+    given): the kernels index these buffers by the grid's sizes, so a host
+    tensor or a short one must fail here, not as a GPU memory fault."""
+    require_cuda(t, name)
+    if tuple(t.shape) != tuple(int(s) for s in shape):
+        raise ValueError(f"{name} must have shape {tuple(int(s) for s in shape)}, got {tuple(t.shape)}")
+    if device is not None and not same_device(t.device, device):
+        raise ValueError(f"{name} lives on {t.device}, expected {device}")
+
+
+def same_device(a, b) -> bool:
+    """Device equality with an index-less "cuda" read as the current device."""
+    a, b = torch.device(a), torch.device(b)
+    if a.type != b.type:
+        return False
+    if a.type != "cuda":
+        return True
+    ia = a.index if a.index is not None else torch.cuda.current_device()
+    ib = b.index if b.index is not None else torch.cuda.current_device()
+    return ia == ib
+
+
 def dims_of_pop(f: torch.Tensor):
     if f.dim() != 4 or f.shape[0] != 19:
         raise ValueError(f"expected populations of shape (19, nx, ny, nz), got {tuple(f.shape)}")
@@ -73,8 +96,8 @@ def new_scalar(nx, ny, nz, dtype, device) -> torch.Tensor:
 def as_field(v, like: torch.Tensor, shape3) -> torch.Tensor:
     """Scalar (Python number or 0-d tensor) or (nx, ny, nz) field -> lattice-storage field."""
     nx, ny, nz = shape3
-    if isinstance(v, torch.Tensor) and v.dim() == 3:
-        require_cuda(v, "field")
+    if isinstance(v, torch.Tensor) and v.dim() > 0:
+        require_shape(v, "field", (nx, ny, nz), like.device)
         return scalar_storage(v.to(like.dtype))
     val = float(v)
     return torch.full((nz, ny, nx), val, dtype=like.dtype, device=like.device)

@@ -98,8 +98,7 @@ def _stream_fwd(f: torch.Tensor, q: torch.Tensor, periodic_yz: bool = True) -> t
     F.require_cuda(f, "f")
     F.require_cuda(q, "q")
     nx, ny, nz = F.dims_of_pop(f)
-    if tuple(q.shape) != tuple(f.shape):
-        raise ValueError(f"q must have the shape of f {tuple(f.shape)}, got {tuple(q.shape)}")
+    F.require_shape(q, "q", f.shape, f.device)
     fs = F.pop_storage(f)
     qs = F.pop_storage(q.to(fs.dtype))
     out = torch.empty_like(fs)
@@ -146,6 +145,8 @@ def _mex(f_post, f_streamed, q, want_pressure):
     F.require_cuda(f_streamed, "f_streamed")
     F.require_cuda(q, "q")
     nx, ny, nz = F.dims_of_pop(f_post)
+    F.require_shape(f_streamed, "f_streamed", f_post.shape, f_post.device)
+    F.require_shape(q, "q", f_post.shape, f_post.device)
     ps = F.pop_storage(f_post)
     ss = F.pop_storage(f_streamed.to(ps.dtype))
     qs = F.pop_storage(q.to(ps.dtype))
@@ -217,6 +218,7 @@ def _hard_bounce_back_fwd(f_collided: torch.Tensor, solid: torch.Tensor) -> torc
     """f[opp] on cells with solid >= 1 (jax_lbm/state.py:146-149)."""
     F.require_cuda(f_collided, "f_collided")
     nx, ny, nz = F.dims_of_pop(f_collided)
+    F.require_shape(solid, "solid", (nx, ny, nz), f_collided.device)
     fs = F.pop_storage(f_collided)
     ss = F.scalar_storage(solid.to(torch.int32))
     out = torch.empty_like(fs)

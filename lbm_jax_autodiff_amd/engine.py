@@ -135,17 +135,22 @@ class Lattice:
         self.inlet_velocity = [float(v) for v in inlet_velocity]
         self.tau = F.scalar_float(tau)
 
+        grid, pops = (self.nx, self.ny, self.nz), (19, self.nx, self.ny, self.nz)
+        for t, name, shape in ((solid, "solid", grid), (porosity, "porosity", grid), (q, "q", pops),
+                               (dqdr, "dqdr", pops)):
+            if t is not None:
+                F.require_shape(t, name, shape, self.device)
         solid_s = F.scalar_storage(solid.to(torch.int32)) if solid is not None else None
         self.solid_storage = solid_s
         self.porosity_storage = None
         if porosity is not None:
-            self.porosity_storage = F.scalar_storage(porosity.detach().to(self.real)).to(self.device)
+            self.porosity_storage = F.scalar_storage(porosity.detach().to(self.real))
         qs = None
         if q is not None:
-            qs = F.pop_storage(q.detach()).to(self.device)
+            qs = F.pop_storage(q.detach())
             if qs.dtype not in (torch.float32, torch.float64):
                 qs = qs.to(torch.float32)
-        dq_s = F.pop_storage(dqdr.detach()).to(self.device) if dqdr is not None else None
+        dq_s = F.pop_storage(dqdr.detach()) if dqdr is not None else None
         self.links = compile_links(qs, solid_s, self.nx, self.ny, self.nz, self.device, dq_s)
 
         d = L.StepDesc()

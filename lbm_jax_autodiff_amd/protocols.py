@@ -217,6 +217,65 @@ def adam(loss_and_grad, r0, n_iters, lr, r_min=0.3, r_max=0.8, b1=0.9, b2=0.999,
     return r, hist
 
 
+# ------------------------------------------- published MRT-path studies --
+
+SHARPNESS_ALPHAS = (5.0, 10.0, 20.0, 40.0, 80.0)
+
+
+def sharpness_case(alpha, *, storage="f32", radius=0.5, n_steps=2880, fd_eps=None, nx=96, ny=48, nz=48):
+    """examples/12_sharpness_sensitivity.py:53-105: 96x48x48, Re = 50 (tau from
+    D = nx/8 = 12), MRT, soft porosity sigma(alpha sdf) with ray q(r), centre
+    world 0, f_0 = f_eq(1, (U, 0, 0)), no guard, 2880 steps; the loss is the
+    Cd of the last (f_post, f_final) pair, and dCd/dr is in world units.
+    ``fd_eps`` adds the script's central difference (r +- eps) of the same
+    forward.  The geometry is computed from float32 coordinates for every
+    storage, as the script's (jnp.arange float32, smooth.py:62-71), so a
+    float64 run differs from the float32 one in the populations only."""
+    res = sphere_loss_and_grad(nx, ny, nz, n_steps, (0.0, 0.0, 0.0), radius, avg_last=1, q_mode="bouzidi",
+                               porosity_mode="soft", sharpness=float(alpha), collision="mrt", re=50.0,
+                               storage=storage, checkpoint="all", guard="none", init_u=0.05,
+                               geometry_dtype=torch.float32)
+    out = {"alpha": float(alpha), "cd": res["loss"], "grad_ad": res["dloss_dr_world"], "tau": res["tau"],
+           "fwd_adj_s": res["fwd_adj_s"]}
+    if fd_eps:
+        kw = dict(avg_last=1, q_mode="bouzidi", porosity_mode="soft", sharpness=float(alpha), collision="mrt",
+                  re=50.0, storage=storage, checkpoint="all", guard="none", init_u=0.05,
+                  geometry_dtype=torch.float32)
+        lp = sphere_loss_and_grad(nx, ny, nz, n_steps, (0.0, 0.0, 0.0), radius + fd_eps, **kw)["loss"]
+        lm = sphere_loss_and_grad(nx, ny, nz, n_steps, (0.0, 0.0, 0.0), radius - fd_eps, **kw)["loss"]
+        out["grad_fd"] = (lp - lm) / (2.0 * fd_eps)
+        out["fd_eps"] = fd_eps
+    return out
+
+
+FOUR_MODES = (("bouzidi", "soft"), ("halfway", "soft"), ("bouzidi", "hard"), ("halfway", "hard"))
+
+
+def four_mode_case(q_mode, porosity_mode, *, storage="f32", radius=0.5, n_steps=2000, iters=0, lr=3e-3):
+    """examples/11_optimisation_v2.py:56-204: 96x48x48, Re = 50, MRT, sharpness
+    20, q ray (bouzidi) or 1/2 (halfway), porosity soft or the 0/1 mask
+    (hard), f_0 = f_eq(1, (U, 0, 0)), 2000 steps, loss = Cd of the last
+    (f_post, f_final) pair.  ``iters`` > 0 runs the script's Adam (optax.adam
+    update, r clipped to [0.3, 0.8]) and reports base, best and final values."""
+
+    def lg(r):
+        out = sphere_loss_and_grad(96, 48, 48, n_steps, (0.0, 0.0, 0.0), r, avg_last=1, q_mode=q_mode,
+                                   porosity_mode=porosity_mode, sharpness=20.0, collision="mrt", re=50.0,
+                                   storage=storage, checkpoint="all", guard="none", init_u=0.05,
+                                   geometry_dtype=torch.float32)
+        return out["loss"], out["dloss_dr_world"]
+
+    cd0, g0 = lg(radius)
+    out = {"mode": f"{q_mode}_{porosity_mode}", "base_cd": cd0, "base_grad": g0}
+    if iters > 0:
+        r_end, hist = adam(lg, radius, iters, lr)
+        cds = [h["loss"] for h in hist]
+        out.update({"iters": iters, "lr": lr, "best_cd": min(cds), "final_r": hist[-1]["r"], "r_after": r_end,
+                    "avg_abs_grad": sum(abs(h["grad"]) for h in hist) / len(hist),
+                    "history": [(round(h["r"], 6), round(h["loss"], 6), round(h["grad"], 6)) for h in hist]})
+    return out
+
+
 # -------------------------------------------------------------------- CLI --
 
 def _emit(d):
@@ -225,7 +284,8 @@ def _emit(d):
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("config", choices=["c1", "c2", "c3", "c3fd", "c3fdh", "c3adam", "c4", "c5", "ex10"])
+    ap.add_argument("config", choices=["c1", "c2", "c3", "c3fd", "c3fdh", "c3adam", "c4", "c5", "ex10", "ex11",
+                                       "ex12"])
     ap.add_argument("--grid", default="64")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--storage", default="f32")
@@ -233,6 +293,8 @@ def main(argv=None):
     ap.add_argument("--adjoint-storage", default=None, choices=["f32", "f16"])
     ap.add_argument("--mode", default="hard")
     ap.add_argument("--repeat", type=int, default=1, help="c3/c3fd: gradient evaluations in one process")
+    ap.add_argument("--iters", type=int, default=0, help="ex11: Adam iterations (the script runs 60)")
+    ap.add_argument("--fd-eps", type=float, default=0.0, help="ex12: central-difference step (the script: 1e-3)")
     args = ap.parse_args(argv)
     if args.config == "c1":
         _emit(channel_flow(n_steps=args.steps or 5000, storage=args.storage))
@@ -286,6 +348,19 @@ def main(argv=None):
         res["config"] = "ex10-" + args.grid
         res["cd_snapshots"] = [round(float(v), 6) for v in cd[499::500].cpu()]
         _emit(res)
+    elif args.config == "ex12":
+        for a in SHARPNESS_ALPHAS:
+            res = sharpness_case(a, storage=args.storage, n_steps=args.steps or 2880, fd_eps=args.fd_eps or None)
+            res["config"] = "ex12"
+            res["storage"] = args.storage
+            _emit(res)
+    elif args.config == "ex11":
+        for qm, pm in FOUR_MODES:
+            res = four_mode_case(qm, pm, storage=args.storage, n_steps=args.steps or 2000,
+                                 iters=0 if (qm, pm) == ("halfway", "hard") else args.iters)
+            res["config"] = "ex11"
+            res["storage"] = args.storage
+            _emit(res)
     elif args.config == "c3adam":
         nx, ny, nz = 96, 48, 48
         c = world_center(nx, ny, nz, (24, 24, 24))

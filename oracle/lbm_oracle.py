@@ -37,6 +37,7 @@ reference publishes.
 
 from __future__ import annotations
 
+import functools
 import math
 from typing import NamedTuple, Optional, Sequence
 
@@ -103,6 +104,7 @@ def mrt_rates_fixed(dtype=torch.float64, faithful32: bool = True) -> torch.Tenso
     return torch.tensor(vals, dtype=dtype)
 
 
+@functools.lru_cache(maxsize=None)
 def mrt_matrix(dtype=torch.float64) -> torch.Tensor:
     """The d'Humieres D3Q19 moment matrix, built from its defining polynomials.
 
@@ -110,6 +112,7 @@ def mrt_matrix(dtype=torch.float64) -> torch.Tensor:
     3pxx, 3pixx, pww, piww, pxy, pyz, pxz, mx, my, mz).  Built here from the
     polynomials of (cx, cy, cz) rather than from the reference's unrolled sums
     (lattice.py:184-209); tests check it against the reference's row-norm table.
+    Cached per dtype: callers only read it.
     """
     rows = []
     for k in range(Q):

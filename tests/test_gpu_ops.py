@@ -120,6 +120,39 @@ def test_momentum_exchange(dev, dt):
                   torch.zeros(3)) == 0.0
 
 
+def test_momentum_exchange_decomposed_invariants(dev, dt):
+    """tests/test_forces.py:45-56 (friction = total - pressure = 0 at equilibrium)
+    and the closed-form pressure part of a plane of direction-1 links:
+    64 (f_eq_1 + f_eq_2)(P) = 64 * 2 w_1 rho0 (1 + 3 U^2) along x for any
+    f_streamed, with P = f_eq(rho0, (U, 0, 0)) + a zero-moment shear."""
+    nx, ny, nz = 16, 8, 8
+    q = -torch.ones((19, nx, ny, nz), dtype=torch.float64)
+    q[1, 7] = 0.5
+    u = torch.zeros((3, nx, ny, nz), dtype=torch.float64)
+    u[0] = 0.05
+    f = lbm.equilibrium(torch.ones((nx, ny, nz), dtype=dt, device=dev), u.to(dt).to(dev))
+    tot, pre = lbm.momentum_exchange_decomposed(f, f, q.to(dt).to(dev))
+    assert float(tot[0]) > 1.0
+    # the reference's float32-rounded weights (lattice.py:53-60) sum to 1 + 3e-9,
+    # so rho and u of f_eq(1, u) are off by that and friction is 1.06e-7 here
+    # (the oracle with the same weights gives the same value)
+    assert maxabs(tot - pre, torch.zeros(3)) < 2e-7 + 10 * TOL[dt]
+    rho0 = 1.02
+    P = O.equilibrium(torch.full((nx, ny, nz), rho0, dtype=torch.float64), u, faithful32=False)
+    P[7] += 1e-3
+    P[10] += 1e-3
+    P[8] -= 1e-3
+    P[9] -= 1e-3
+    g = torch.Generator().manual_seed(4)
+    S = P + 1e-2 * torch.rand(P.shape, generator=g, dtype=torch.float64)
+    tot, pre = lbm.momentum_exchange_decomposed(P.to(dt).to(dev), S.to(dt).to(dev), q.to(dt).to(dev))
+    # the kernels use the reference's float32-rounded weights (lattice.py:53-60)
+    w1 = float(O.weights(torch.float64)[1])
+    assert abs(float(pre[0]) - 64.0 * 2.0 * w1 * rho0 * (1.0 + 3.0 * 0.05 ** 2)) < 100 * TOL[dt]
+    assert float(cpu(pre[1:]).abs().max()) < 100 * TOL[dt]
+    assert abs(float(tot[0]) - float((P[1, 7] + S[2, 7]).sum())) < 100 * TOL[dt]
+
+
 def test_projected_area_and_cd(dev):
     solid = torch.zeros((16, 16, 16), dtype=torch.int32)
     solid[6:10, 6:10, 6:10] = 1  # box 4x4 in the y-z shadow (test_forces.py:62-67)
