@@ -35,12 +35,27 @@ import time
 import torch
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the float32 fused step on this workload, from the
-# FETCH_SIZE / WRITE_SIZE counter passes scaled by a 1 GiB copy
-# (profiles/k_step_mrt_f32_256x128x128_hbm_traffic.md); re-measure when the
-# kernel changes
-MEASURED_TRAFFIC_BYTES = 654.5e6
 NX, NY, NZ = 256, 128, 128
+# HBM bytes per launch of the float32 fused step on this workload come from the
+# FETCH_SIZE / WRITE_SIZE counter passes (scaled by a 1 GiB copy) summarised in
+# this committed profile; its "traffic_bytes_per_launch" line names the kernel
+# instance it was measured on
+TRAFFIC_PROFILE = os.path.join("profiles", "k_step_mrt_f32_256x128x128_hbm_traffic.md")
+
+
+def measured_traffic():
+    """(bytes per launch, kernel name, source file) from the traffic profile, or
+    (None, None, file) when the file or its line is missing."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), TRAFFIC_PROFILE)
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    fields = dict(kv.split("=", 1) for kv in line.split(":", 1)[1].split())
+                    return float(fields["bytes"]), fields["kernel"], TRAFFIC_PROFILE
+    except OSError:
+        pass
+    return None, None, TRAFFIC_PROFILE
 
 
 def log(msg):
@@ -341,6 +356,7 @@ def main():
         if args.study_steps > 0:
             extras["dcd_dr_study"] = dcd_dr_study(args.study_steps, dev)
 
+    traffic, traffic_kernel, traffic_src = measured_traffic()
     out = {
         "metric": "fwd & fwd+adjoint MLUPS, % HBM roofline; dCd/dr time at <=1e-3 rel err",
         "value": round(mlups, 1),
@@ -358,7 +374,8 @@ def main():
                                "fused forward step", "grid": [NX, NY, NZ], "tau": round(tau, 6),
                    "cells": cells, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": MEASURED_TRAFFIC_BYTES,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": traffic_src, "traffic_kernel": traffic_kernel,
                      "kernel_ms": round(loop_ms, 5), "single_launch_ms": round(k_ms, 5),
                      "bytes_per_cell_update": bytes_per_lu},
         "extras": extras,

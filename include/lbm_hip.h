@@ -92,6 +92,12 @@ int lbm_soft_bounce_back(int dtype, const void* f, const void* porosity, void* f
 int lbm_hard_bounce(int dtype, const void* f, const int32_t* solid, void* f_out,
                     int nx, int ny, int nz, lbm_stream_t stream);
 
+/* Post-step stability guard: cells with rho < 0.3, rho > 2, |u| > 0.4 or
+ * rho = NaN are reset to f_eq(1, 0), the others copied.  Replaces the inline
+ * guard of step() at jax_lbm/state.py:158-169. */
+int lbm_stability_guard(int dtype, const void* f, void* f_out, int nx, int ny, int nz,
+                        lbm_stream_t stream);
+
 /* ---- Streaming (jax_lbm/core/streaming.py) -------------------------------- */
 
 /* Pull streaming with Bouzidi links from a dense q (19N, same dtype; q<0 = no
@@ -380,6 +386,11 @@ int lbm_soft_bounce_back_vjp(int dtype, const void* f, const void* porosity, con
 /* hard bounce where solid >= 1 (jax_lbm/state.py:146-149). */
 int lbm_hard_bounce_vjp(int dtype, const int32_t* solid, const void* g_out, void* g_f, int nx,
                         int ny, int nz, lbm_stream_t stream);
+
+/* stability guard (jax_lbm/state.py:158-169): g_f = g_out where the guard
+ * input f is stable, 0 at reset cells (the reset value is a constant). */
+int lbm_stability_guard_vjp(int dtype, const void* f, const void* g_out, void* g_f, int nx,
+                            int ny, int nz, lbm_stream_t stream);
 
 /* stream with Bouzidi links (jax_lbm/core/streaming.py:40-185): g_f, and
  * g_q (optional; needs f) on the taken branch inside the clip bounds. */

@@ -59,6 +59,8 @@ _SIGS = {
                                     c_void_p]),
     "lbm_soft_bounce_back": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
     "lbm_hard_bounce": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
+    "lbm_stability_guard": (c_int, [c_int, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
+    "lbm_stability_guard_vjp": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
     "lbm_stream": (c_int, [c_int, c_void_p, c_void_p, c_int, c_void_p, c_int, c_int, c_int, c_void_p]),
     "lbm_zou_he_inlet": (c_int, [c_int, c_void_p, c_double, c_double, c_double, c_int, c_double, c_int, c_int,
                                  c_int, c_void_p]),

@@ -230,7 +230,7 @@ class _RolloutFn(torch.autograd.Function):
         ctx.f0_dtype = f0.dtype
         ctx.q_dtype = q.dtype if q is not None else None
         ctx.phi_dtype = porosity.dtype if porosity is not None else None
-        ctx.u_dtype = u_in.dtype
+        ctx.u_dtype, ctx.u_device = u_in.dtype, u_in.device
         s_out = lat.from_state(s_final)
         p_out = lat.from_state(p_last)
         if lat.storage in ("f32", "f64"):
@@ -336,7 +336,8 @@ class _RolloutFn(torch.autograd.Function):
             dense[1:, :].index_copy_(1, cells, dq.to(ctx.q_dtype))
             g_q = F.pop_view(dense.reshape(19, nz, ny, nx))
         g_phi = F.scalar_view(phi_acc.reshape(nz, ny, nx)).to(ctx.phi_dtype) if want_phi else None
-        g_u = u_acc.sum(0).to(ctx.u_dtype) if want_u else None
+        # on the velocity's own device: a host tensor with requires_grad is accepted
+        g_u = u_acc.sum(0).to(device=ctx.u_device, dtype=ctx.u_dtype) if want_u else None
         return g_f0, g_q, g_phi, g_u, None
 
 

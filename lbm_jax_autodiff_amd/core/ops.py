@@ -227,6 +227,17 @@ def _hard_bounce_back_fwd(f_collided: torch.Tensor, solid: torch.Tensor) -> torc
     return F.pop_view(out)
 
 
+def _stability_guard_fwd(f: torch.Tensor) -> torch.Tensor:
+    """Reset unstable cells to f_eq(1, 0) (jax_lbm/state.py:158-169)."""
+    F.require_cuda(f, "f")
+    nx, ny, nz = F.dims_of_pop(f)
+    fs = F.pop_storage(f)
+    out = torch.empty_like(fs)
+    _launch("lbm_stability_guard", F.dtype_tag(fs.dtype), L.ptr(fs), L.ptr(out), nx, ny, nz,
+            L.stream_handle(fs.device))
+    return F.pop_view(out)
+
+
 # ---------------------------------------------------------------------------
 # Reverse mode.  Each public operator is a torch.autograd.Function whose
 # backward calls the operator's hand-written VJP kernel (lbm_*_vjp), so
@@ -245,7 +256,7 @@ def _field_grad(g_storage: torch.Tensor, like):
         return None
     if like.dim() == 3:
         return F.scalar_view(g_storage).to(like.dtype)
-    return g_storage.sum().reshape(like.shape).to(like.dtype)
+    return g_storage.sum().reshape(like.shape).to(like.device, like.dtype)
 
 
 def _pop_grad(g: torch.Tensor, dtype) -> torch.Tensor:
@@ -409,8 +420,9 @@ class _ZouHeInlet(torch.autograd.Function):
         rt = F.scalar_float(ctx.rt_like) if has_rt else 0.0
         _launch("lbm_zou_he_inlet_vjp", F.dtype_tag(fs.dtype), L.ptr(fs), *ctx.vel, int(has_rt), rt, L.ptr(gs),
                 L.ptr(gu), L.ptr(grt), nx, ny, nz, L.stream_handle(fs.device))
-        g_u = gu.sum(0).reshape(ctx.vel_like.shape).to(ctx.vel_like.dtype) if want_u else None
-        g_rt = grt.sum().reshape(ctx.rt_like.shape).to(ctx.rt_like.dtype) if want_rt else None
+        # returned on the input's own device (a host velocity tensor is allowed)
+        g_u = gu.sum(0).reshape(ctx.vel_like.shape).to(ctx.vel_like.device, ctx.vel_like.dtype) if want_u else None
+        g_rt = grt.sum().reshape(ctx.rt_like.shape).to(ctx.rt_like.device, ctx.rt_like.dtype) if want_rt else None
         return F.pop_view(gs), g_u, g_rt
 
 
@@ -430,7 +442,7 @@ class _ZouHeOutlet(torch.autograd.Function):
         grho = torch.empty(ny * nz, dtype=torch.float64, device=gs.device) if want else None
         _launch("lbm_zou_he_outlet_vjp", F.dtype_tag(gs.dtype), F.scalar_float(ctx.rho_like), L.ptr(gs), L.ptr(grho),
                 nx, ny, nz, L.stream_handle(gs.device))
-        g_rho = grho.sum().reshape(ctx.rho_like.shape).to(ctx.rho_like.dtype) if want else None
+        g_rho = grho.sum().reshape(ctx.rho_like.shape).to(ctx.rho_like.device, ctx.rho_like.dtype) if want else None
         return F.pop_view(gs), g_rho
 
 
@@ -530,6 +542,24 @@ class _HardBounce(torch.autograd.Function):
         return F.pop_view(gf), None
 
 
+class _Guard(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, f):
+        ctx.save_for_backward(f)
+        return _stability_guard_fwd(f)
+
+    @staticmethod
+    def backward(ctx, g):
+        (f,) = ctx.saved_tensors
+        nx, ny, nz = F.dims_of_pop(f)
+        fs = F.pop_storage(f)
+        gs = _pop_grad(g, fs.dtype)
+        gf = torch.empty_like(fs)
+        _launch("lbm_stability_guard_vjp", F.dtype_tag(fs.dtype), L.ptr(fs), L.ptr(gs), L.ptr(gf), nx, ny, nz,
+                L.stream_handle(fs.device))
+        return F.pop_view(gf).to(f.dtype)
+
+
 def compute_density(f: torch.Tensor) -> torch.Tensor:
     """rho = sum_i f_i.  jax_lbm/core/equilibrium.py:22-48."""
     return _Density.apply(f)
@@ -604,3 +634,10 @@ def soft_bounce_back(f_collided: torch.Tensor, porosity: torch.Tensor) -> torch.
 def hard_bounce_back(f_collided: torch.Tensor, solid: torch.Tensor) -> torch.Tensor:
     """f[opp] on cells with solid >= 1 (jax_lbm/state.py:146-149)."""
     return _HardBounce.apply(f_collided, solid)
+
+
+def stability_guard(f: torch.Tensor) -> torch.Tensor:
+    """Cells with rho < 0.3, rho > 2, |u| > 0.4 or rho = NaN become f_eq(1, 0);
+    the guard that ends step() at jax_lbm/state.py:158-169.  Reset cells pass
+    no cotangent (the reset value is a constant)."""
+    return _Guard.apply(f)

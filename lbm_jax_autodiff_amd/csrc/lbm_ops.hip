@@ -123,6 +123,18 @@ __global__ void k_hard_bounce(const R* __restrict__ fin, const int32_t* __restri
   store_cell(fout, N, c, f);
 }
 
+// Post-step stability guard (state.py:158-169): cells with rho < 0.3,
+// rho > 2, |u| > 0.4 or rho = NaN become f_eq(1, 0) = w.
+template <typename R>
+__global__ void k_guard(const R* __restrict__ fin, R* __restrict__ fout, size_t N) {
+  const size_t c = blockIdx.x * size_t(blockDim.x) + threadIdx.x;
+  if (c >= N) return;
+  R f[Q];
+  load_cell(fin, N, c, f);
+  if (guard_bad(f)) set_weights(f);
+  store_cell(fout, N, c, f);
+}
+
 // Pull streaming + Bouzidi (streaming.py:99-183), dense q.
 template <typename R>
 __global__ void k_stream(const R* __restrict__ f, const R* __restrict__ q, int periodic,
@@ -577,6 +589,14 @@ int lbm_hard_bounce(int dtype, const void* f, const int32_t* solid, void* f_out,
   const size_t N = size_t(nx) * ny * nz;
   LBM_DISPATCH_RF(dtype, hipLaunchKernelGGL(k_hard_bounce<R>, grid_for(N), kBlock, 0, (hipStream_t)stream,
                                             (const R*)f, solid, (R*)f_out, N));
+  return launch_status();
+}
+
+int lbm_stability_guard(int dtype, const void* f, void* f_out, int nx, int ny, int nz, lbm_stream_t stream) {
+  if (!dims_ok(nx, ny, nz) || !f || !f_out) return LBM_E_ARG;
+  const size_t N = size_t(nx) * ny * nz;
+  LBM_DISPATCH_RF(dtype, hipLaunchKernelGGL(k_guard<R>, grid_for(N), kBlock, 0, (hipStream_t)stream,
+                                            (const R*)f, (R*)f_out, N));
   return launch_status();
 }
 

@@ -170,6 +170,23 @@ __global__ void k_hard_bounce_vjp(const int32_t* __restrict__ solid, const R* __
   vstore(gf, N, c, g);
 }
 
+// Stability guard (state.py:158-169): where(unstable, f_eq(1, 0), f) sends the
+// cotangent to the reset branch at unstable cells, and that branch is a
+// constant there (where(unstable, 1, rho), where(unstable, 0, u)), so those
+// cells pass none; the predicate is recomputed from the guard's input.
+template <typename R>
+__global__ void k_guard_vjp(const R* __restrict__ fin, const R* __restrict__ gout, R* __restrict__ gf, size_t N) {
+  const size_t c = blockIdx.x * size_t(blockDim.x) + threadIdx.x;
+  if (c >= N) return;
+  R f[Q];
+  vload(fin, N, c, f);
+  const bool bad = guard_bad(f);
+  sfor<Q>([&](auto I) {
+    constexpr int i = I;
+    gf[i * N + c] = bad ? R(0) : gout[i * N + c];
+  });
+}
+
 // Bouzidi weights of the two branches (streaming.py:155-178), with their
 // clips: A = 1 / (2 clip(q, 1/2, 1)) for q >= 1/2, B = 2 clip(q, 0, 1/2) below.
 template <typename R>
@@ -561,6 +578,15 @@ int lbm_hard_bounce_vjp(int dtype, const int32_t* solid, const void* g_out, void
   const size_t N = size_t(nx) * ny * nz;
   LBM_VJP_DISPATCH(dtype, hipLaunchKernelGGL(k_hard_bounce_vjp<R>, vjp_grid(N), kVjpBlock, 0, (hipStream_t)stream,
                                              solid, (const R*)g_out, (R*)g_f, N));
+  return vjp_status();
+}
+
+int lbm_stability_guard_vjp(int dtype, const void* f, const void* g_out, void* g_f, int nx, int ny, int nz,
+                            lbm_stream_t stream) {
+  if (!vjp_dims_ok(nx, ny, nz) || !f || !g_out || !g_f) return LBM_E_ARG;
+  const size_t N = size_t(nx) * ny * nz;
+  LBM_VJP_DISPATCH(dtype, hipLaunchKernelGGL(k_guard_vjp<R>, vjp_grid(N), kVjpBlock, 0, (hipStream_t)stream,
+                                             (const R*)f, (const R*)g_out, (R*)g_f, N));
   return vjp_status();
 }
 

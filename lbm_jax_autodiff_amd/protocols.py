@@ -222,25 +222,28 @@ def adam(loss_and_grad, r0, n_iters, lr, r_min=0.3, r_max=0.8, b1=0.9, b2=0.999,
 SHARPNESS_ALPHAS = (5.0, 10.0, 20.0, 40.0, 80.0)
 
 
-def sharpness_case(alpha, *, storage="f32", radius=0.5, n_steps=2880, fd_eps=None, nx=96, ny=48, nz=48):
+def sharpness_case(alpha, *, storage="f32", radius=0.5, n_steps=2880, fd_eps=None, nx=96, ny=48, nz=48,
+                   geometry_dtype=torch.float32):
     """examples/12_sharpness_sensitivity.py:53-105: 96x48x48, Re = 50 (tau from
     D = nx/8 = 12), MRT, soft porosity sigma(alpha sdf) with ray q(r), centre
     world 0, f_0 = f_eq(1, (U, 0, 0)), no guard, 2880 steps; the loss is the
     Cd of the last (f_post, f_final) pair, and dCd/dr is in world units.
     ``fd_eps`` adds the script's central difference (r +- eps) of the same
     forward.  The geometry is computed from float32 coordinates for every
-    storage, as the script's (jnp.arange float32, smooth.py:62-71), so a
-    float64 run differs from the float32 one in the populations only."""
+    storage by default, as the script's (jnp.arange float32, smooth.py:62-71),
+    so a float64 run differs from the float32 one in the populations only;
+    ``geometry_dtype`` float64 takes r and the coordinates in float64 (for
+    central differences in r at small steps)."""
     res = sphere_loss_and_grad(nx, ny, nz, n_steps, (0.0, 0.0, 0.0), radius, avg_last=1, q_mode="bouzidi",
                                porosity_mode="soft", sharpness=float(alpha), collision="mrt", re=50.0,
                                storage=storage, checkpoint="all", guard="none", init_u=0.05,
-                               geometry_dtype=torch.float32)
+                               geometry_dtype=geometry_dtype)
     out = {"alpha": float(alpha), "cd": res["loss"], "grad_ad": res["dloss_dr_world"], "tau": res["tau"],
            "fwd_adj_s": res["fwd_adj_s"]}
     if fd_eps:
         kw = dict(avg_last=1, q_mode="bouzidi", porosity_mode="soft", sharpness=float(alpha), collision="mrt",
                   re=50.0, storage=storage, checkpoint="all", guard="none", init_u=0.05,
-                  geometry_dtype=torch.float32)
+                  geometry_dtype=geometry_dtype)
         lp = sphere_loss_and_grad(nx, ny, nz, n_steps, (0.0, 0.0, 0.0), radius + fd_eps, **kw)["loss"]
         lm = sphere_loss_and_grad(nx, ny, nz, n_steps, (0.0, 0.0, 0.0), radius - fd_eps, **kw)["loss"]
         out["grad_fd"] = (lp - lm) / (2.0 * fd_eps)
@@ -294,8 +297,14 @@ def main(argv=None):
     ap.add_argument("--mode", default="hard")
     ap.add_argument("--repeat", type=int, default=1, help="c3/c3fd: gradient evaluations in one process")
     ap.add_argument("--iters", type=int, default=0, help="ex11: Adam iterations (the script runs 60)")
+    ap.add_argument("--offset", default="inlet",
+                    help="c4/c5 half storage: 'inlet' (f_eq(1, U)), 'weights' (w_i) or a number k (f_eq(1, k U))")
     ap.add_argument("--fd-eps", type=float, default=0.0, help="ex12: central-difference step (the script: 1e-3)")
     args = ap.parse_args(argv)
+    offset = args.offset
+    if offset not in ("inlet", "weights"):
+        from .engine import equilibrium_values
+        offset = equilibrium_values(1.0, (float(offset) * 0.05, 0.0, 0.0))
     if args.config == "c1":
         _emit(channel_flow(n_steps=args.steps or 5000, storage=args.storage))
     elif args.config == "c2":
@@ -381,7 +390,7 @@ def main(argv=None):
             ck = parts[0] if len(parts) == 1 else tuple(parts)
         res = sphere_loss_and_grad(nx, ny, nz, steps, (0.0, 0.0, 0.0), 0.5, storage=args.storage,
                                    checkpoint=ck, avg_last=int(0.3 * steps), guard="pre",
-                                   adjoint_storage=args.adjoint_storage)
+                                   adjoint_storage=args.adjoint_storage, offset=offset)
         res["config"] = "c4"
         _emit(res)
     elif args.config == "c5":
@@ -397,8 +406,10 @@ def main(argv=None):
         res = sphere_loss_and_grad(nx, ny, nz, steps, (0.0, 0.0, 0.0), 0.5, storage=args.storage,
                                    checkpoint=ck, avg_last=int(0.3 * steps), guard="pre",
                                    porosity_mode="soft" if soft else "hard", sharpness=10.0,
-                                   smagorinsky_cs=0.1 if soft else None, adjoint_storage=args.adjoint_storage)
+                                   smagorinsky_cs=0.1 if soft else None, adjoint_storage=args.adjoint_storage,
+                                   offset=offset)
         res["config"] = "c5-" + args.mode
+        res["offset"] = args.offset
         _emit(res)
 
 

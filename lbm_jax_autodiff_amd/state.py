@@ -6,11 +6,19 @@ outlet -> post-step stability guard (state.py:129-171) — as two launches of
 the fused kernel (collide, then stream).  ``simulate`` runs n steps with one
 launch per step.
 
-When the populations, q or a tensor inlet velocity require grad, both run
-through the differentiable fused rollout (hand-written adjoint), with
-periodic y/z as the adjoint requires.  ``checkpoint`` then picks its memory
-policy, as jax.checkpoint does in the reference: True recomputes segments of
-about sqrt(n) steps, False keeps every state.
+Reverse mode (the reference differentiates step/simulate with jax.grad
+through every LBMParams leaf):
+* when the populations, q or a tensor inlet velocity require grad, with
+  periodic y/z and a scalar tau that does not, both run through the
+  differentiable fused rollout (hand-written adjoint of the whole step);
+* when tau requires grad or is a per-cell field, or y/z are clamped
+  (periodic_yz=False, streaming.py:134-142), they run the same step as a
+  composition of the per-operator kernels, each with its hand-written VJP
+  kernel (and a per-cell tau runs that way forward too, since the fused
+  step takes one tau).
+``checkpoint`` picks the memory policy, as jax.checkpoint does in the
+reference: True recomputes segments of about sqrt(n) steps, False keeps
+every state.
 """
 
 from __future__ import annotations
@@ -19,8 +27,10 @@ import math
 from typing import NamedTuple
 
 import torch
+from torch.utils.checkpoint import checkpoint as _recompute
 
 from . import _field as F
+from .core import ops as _ops
 from .engine import Lattice
 
 
@@ -57,20 +67,69 @@ def _lattice(state: LBMState, params: LBMParams) -> Lattice:
                    storage=storage, device=state.f.device)
 
 
+def _tau_is_field(params: LBMParams) -> bool:
+    return isinstance(params.tau, torch.Tensor) and params.tau.dim() > 0 and params.tau.numel() > 1
+
+
+def _tau_needs_grad(params: LBMParams) -> bool:
+    return isinstance(params.tau, torch.Tensor) and params.tau.requires_grad
+
+
 def _wants_grad(state: LBMState, params: LBMParams) -> bool:
     if not torch.is_grad_enabled():
         return False
     vel = params.inlet_velocity
     return bool(state.f.requires_grad or (state.q is not None and state.q.requires_grad)
-                or (isinstance(vel, torch.Tensor) and vel.requires_grad))
+                or (isinstance(vel, torch.Tensor) and vel.requires_grad) or _tau_needs_grad(params))
+
+
+def _per_op_path(state: LBMState, params: LBMParams, grad: bool) -> bool:
+    """Per-operator composition: a per-cell tau (any mode), or in reverse mode
+    a tau that requires grad or clamped y/z (the fused adjoint covers the rest)."""
+    if _tau_is_field(params):
+        return True
+    return grad and (_tau_needs_grad(params) or not bool(params.periodic_yz))
+
+
+def _step_ops(f: torch.Tensor, state: LBMState, params: LBMParams) -> torch.Tensor:
+    """One step as differentiable per-operator kernels, in the order of
+    jax_lbm/state.py:129-169: [Smagorinsky tau] -> BGK|MRT -> hard bounce
+    (solid >= 1) -> stream with Bouzidi -> Zou-He inlet -> Zou-He outlet ->
+    stability guard."""
+    tau = params.tau
+    if int(params.use_smagorinsky):
+        tau = _ops.smagorinsky_tau(f, tau, float(params.smagorinsky_cs))
+    f = _ops.mrt(f, tau) if int(params.collision_mode) == 1 else _ops.bgk(f, tau)
+    f = _ops.hard_bounce_back(f, state.solid)
+    f = _ops.stream(f, state.q, bool(params.periodic_yz))
+    f = _ops.zou_he_inlet(f, params.inlet_velocity)
+    f = _ops.zou_he_outlet(f, 1.0)
+    return _ops.stability_guard(f)
+
+
+def _ops_rollout(state: LBMState, params: LBMParams, n: int, checkpoint: bool) -> torch.Tensor:
+    """n steps of ``_step_ops``; with checkpoint each segment of about sqrt(n)
+    steps keeps only its input and is recomputed in the backward."""
+
+    def segment(f, k):
+        for _ in range(k):
+            f = _step_ops(f, state, params)
+        return f
+
+    seg = max(1, int(math.ceil(math.sqrt(n)))) if checkpoint and n > 1 else n
+    f, t = state.f, 0
+    while t < n:
+        k = min(seg, n - t)
+        if seg < n and torch.is_grad_enabled():
+            f = _recompute(segment, f, k, use_reentrant=False)
+        else:
+            f = segment(f, k)
+        t += k
+    return f
 
 
 def _rollout(state: LBMState, params: LBMParams, n: int, checkpoint: bool) -> torch.Tensor:
     from .adjoint import rollout
-    if not bool(params.periodic_yz):
-        raise NotImplementedError("reverse mode of step/simulate needs periodic_yz=True")
-    if isinstance(params.tau, torch.Tensor) and params.tau.requires_grad:
-        raise NotImplementedError("reverse mode of step/simulate does not differentiate tau")
     vel = params.inlet_velocity
     if not isinstance(vel, torch.Tensor):
         vel = [float(v) for v in vel]
@@ -86,7 +145,10 @@ def _rollout(state: LBMState, params: LBMParams, n: int, checkpoint: bool) -> to
 def step(state: LBMState, params: LBMParams) -> LBMState:
     """One timestep.  jax_lbm/state.py:87-171."""
     F.require_cuda(state.f, "state.f")
-    if _wants_grad(state, params):
+    grad = _wants_grad(state, params)
+    if _per_op_path(state, params, grad):
+        return LBMState(_ops_rollout(state, params, 1, False), state.solid, state.q)
+    if grad:
         return LBMState(_rollout(state, params, 1, False), state.solid, state.q)
     lat = _lattice(state, params)
     s_final, _, _ = lat.run(state.f, 1, forces=False)
@@ -98,7 +160,10 @@ def simulate(state: LBMState, params: LBMParams, n_steps: int, checkpoint: bool 
     F.require_cuda(state.f, "state.f")
     if int(n_steps) == 0:
         return state
-    if _wants_grad(state, params):
+    grad = _wants_grad(state, params)
+    if _per_op_path(state, params, grad):
+        return LBMState(_ops_rollout(state, params, int(n_steps), checkpoint), state.solid, state.q)
+    if grad:
         return LBMState(_rollout(state, params, int(n_steps), checkpoint), state.solid, state.q)
     lat = _lattice(state, params)
     s_final, _, _ = lat.run(state.f, int(n_steps), forces=False)

@@ -36,3 +36,11 @@ def cpu(t):
 
 def maxabs(a, b):
     return float((cpu(a).to(torch.float64) - cpu(b).to(torch.float64)).abs().max())
+
+
+def perturbed_start(nx, ny, nz, seed, amp=1e-8):
+    """f_0 = w_i + amp * N(0, 1), the noise drawn in float64 on the CPU from
+    torch.Generator().manual_seed(seed) with shape (19, nx, ny, nz)."""
+    g = torch.Generator().manual_seed(int(seed))
+    w = O.weights(torch.float64).reshape(19, 1, 1, 1)
+    return w + amp * torch.randn((19, nx, ny, nz), generator=g, dtype=torch.float64)

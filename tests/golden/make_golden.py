@@ -4,6 +4,8 @@
     python tests/golden/make_golden.py c1     # 64x21x21 channel, 5000 steps (~10 min)
     python tests/golden/make_golden.py c2     # 64x32x32 sphere Cd, 8000 steps (~15 min)
     python tests/golden/make_golden.py c3     # 96x48x48 dCd/dr, 2000 steps (~40 min)
+    python tests/golden/make_golden.py c2p    # c2 in float32 from 3 perturbed starts (~30 min)
+    python tests/golden/make_golden.py ex12   # examples/12 sharpness sweep forward, float32 (~10 min)
 
 The reference (jax_lbm) cannot be imported here: jax, jaxlib and optax are not
 installed.  ``kats`` therefore holds the values the reference publishes (its
@@ -30,6 +32,9 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 from oracle import lbm_oracle as O  # noqa: E402
+
+sys.path.insert(0, os.path.dirname(HERE))
+from _helpers import perturbed_start  # noqa: E402
 
 
 def _threads():
@@ -77,6 +82,34 @@ def kats():
         "halfway_hard_d_cd_d_r": {"source": "README.md:35; paper/paper.tex:651-662",
                                   "setup": "q = 1/2 everywhere (compute_q_from_solid) with a hard mask: nothing "
                                            "depends on r", "value": 0.0},
+        "sharpness_sweep_96x48x48": {
+            "source": "paper/make_convergence_figures.py:86-89 (A100 runs of examples/12_sharpness_sensitivity.py)",
+            "setup": "examples/12_sharpness_sensitivity.py:53-139: 96x48x48 MRT, Re=50 (tau=float32(0.536)), soft "
+                     "porosity sigmoid(alpha sdf) + ray q(r), centre world 0, r=0.5, f0 = f_eq(1,(0.05,0,0)), no "
+                     "guard, 2880 steps, Cd of the last (f_post, f_final) pair, jax.grad and FD (eps=1e-3) in r; "
+                     "float32",
+            "alpha": [5.0, 10.0, 20.0, 40.0, 80.0], "cd": [3.7555, 1.3538, 0.8745, 0.3283, 0.5414],
+            "grad_ad": [15.1634, 3.6649, 1.8524, 9.4889, 2.4426],
+            "grad_fd": [15.4666, 3.7023, 2.2820, 10.3985, 3.3549], "decimals": 4},
+        "four_mode_96x48x48": {
+            "source": "README.md:31-39; paper/make_convergence_figures.py:131-134 (A100 runs of "
+                      "examples/11_optimisation_v2.py)",
+            "setup": "examples/11_optimisation_v2.py:56-240: 96x48x48 MRT, Re=50, sharpness 20, q ray (bouzidi) or "
+                     "1/2 (halfway), porosity soft or the 0/1 mask (hard), f0 = f_eq(1,(0.05,0,0)), 2000 steps, "
+                     "Cd of the last (f_post, f_final) pair; optax.adam lr=3e-3 from r=0.5, r clipped to "
+                     "[0.3, 0.8], 60 iterations (halfway_hard: one evaluation); float32",
+            "modes": ["halfway_hard", "bouzidi_hard", "halfway_soft", "bouzidi_soft"],
+            "base_cd": [1.85, 1.77, 0.43, 0.45], "best_cd": [None, 1.66, 0.003, 0.002],
+            "avg_abs_grad": [0.0, 5.55, 6.30, 3.19], "final_r": [0.500, 0.314, 0.652, 0.663]},
+        "cd_convergence_ex10": {
+            "source": "paper/make_convergence_figures.py:52-54; README.md:45-52 (A100 runs of "
+                      "examples/10_cd_convergence.py)",
+            "setup": "MRT + hard sphere with ray q, Re=100, centre world 0, r=0.5, f0 = f_eq(1,(0.05,0,0)), "
+                     "pre-collision guard to f_eq(1,(0.05,0,0)), Cd sampled every 500 steps, mean and population "
+                     "std of the last max(3, int(0.3 * samples)); float32",
+            "grids": [[64, 32, 32], [128, 64, 64], [256, 128, 128], [384, 192, 192]],
+            "steps": [8000, 16000, 28000, 40000], "cd_mean": [1.2903, 1.1863, 1.1854, 1.1679],
+            "cd_std": [0.0190, 0.0406, 0.0705, 0.0769]},
     })
 
 
@@ -208,10 +241,94 @@ def c3(steps=2000, avg_last=600, seg=5):
         "cd_tail": [float(v) for v in cd[-avg_last:].detach()]})
 
 
+C2P_SEEDS = (1, 2, 3)
+
+
+def c2p(seeds=C2P_SEEDS, merge=True):
+    """The 64x32x32 sphere of ``c2`` in float32 from three initial states
+    perturbed by 1e-8 (``perturbed_start``): how far float32 rounding alone
+    moves the time-averaged Cd of this flow started from rest."""
+    _threads()
+    nx, ny, nz, T = 64, 32, 32, 8000
+    dt = torch.float32
+    c, tau = _sphere_setup(nx, ny, nz, (16, 16, 16), dt)
+    solid, q = O.create_sphere(nx, ny, nz, c, 0.5)
+    q = q.to(dt)
+    area = float(O.projected_area(solid))
+    rho = torch.ones((nx, ny, nz), dtype=dt)
+    uref = torch.zeros((3, nx, ny, nz), dtype=dt)
+    uref[0] = 0.05
+    kw = dict(tau=torch.tensor(tau, dtype=dt), velocity=torch.tensor([0.05, 0.0, 0.0], dtype=dt), q=q,
+              solid=solid, collision="mrt", guard="pre", f_eq_ref=O.equilibrium(rho, uref))
+    path = os.path.join(HERE, "c2_sphere_cd_64x32x32_perturbed_f32.json")
+    runs = {}
+    if merge and os.path.exists(path):
+        with open(path) as fh:
+            runs = json.load(fh)["runs"]
+    k = max(1, int(round(0.3 * T)))
+    for seed in seeds:
+        f = perturbed_start(nx, ny, nz, seed).to(dt)
+        cd = []
+        t0 = time.perf_counter()
+        with torch.no_grad():
+            for t in range(T):
+                f, fp = O.hot_step(f, **kw)
+                cd.append(float(O.drag_coefficient(O.momentum_exchange(fp, f, q), 0.05, area)))
+        runs[str(seed)] = {"cd_tail30_mean": sum(cd[T - k:]) / k, "cd_every_500": cd[499::500]}
+        print(f"c2p seed {seed}: tail mean {runs[str(seed)]['cd_tail30_mean']:.9f} "
+              f"({time.perf_counter() - t0:.0f} s)", flush=True)
+    _write("c2_sphere_cd_64x32x32_perturbed_f32.json", {
+        "setup": "the c2 workload (c2_sphere_cd_64x32x32.json) in float32 with f0 = w + 1e-8 N(0,1), noise from "
+                 "torch.Generator().manual_seed(seed) in float64, shape (19, nx, ny, nz), cast to float32",
+        "generator": "oracle.lbm_oracle.hot_step + momentum_exchange (float32)", "steps": T, "amp": 1e-8,
+        "tail_steps": k, "runs": runs})
+
+
+SHARPNESS_ALPHAS = (5.0, 10.0, 20.0, 40.0, 80.0)
+
+
+def ex12(alphas=SHARPNESS_ALPHAS, n_steps=2880):
+    """examples/12_sharpness_sensitivity.py:53-79 forward in float32: 96x48x48,
+    Re = 50 (tau = float32(3 * 0.05 * 12 / 50 + 1/2)), MRT, soft porosity
+    sigmoid(alpha sdf) with ray q(r) (float32 coordinates), centre world 0,
+    r = 0.5, f0 = f_eq(1, (0.05, 0, 0)), no guard, 2880 steps; Cd of the last
+    (f_post, f_final) pair with area = max(A, 1)."""
+    _threads()
+    nx, ny, nz = 96, 48, 48
+    dt = torch.float32
+    u_in = 0.05
+    tau = torch.tensor(3.0 * (u_in * (nx * 0.5 / 4.0) / 50.0) + 0.5, dtype=dt)
+    out = {"setup": "96x48x48 D3Q19 MRT tau=float32(0.536) (Re=50, nominal D=12), soft porosity sigmoid(alpha sdf) "
+                    "with ray q(r) from smooth_sphere_geometry (float32 coordinates), centre world 0, r=0.5 world, "
+                    "periodic y/z, Zou-He inlet u=0.05 / outlet rho=1, f0 = f_eq(1,(0.05,0,0)), no guard, 2880 "
+                    "steps; Cd of the last (f_post, f_final) pair, area = max(A, 1)",
+           "generator": "oracle.lbm_oracle (float32)", "steps": n_steps, "cd": {}}
+    for a in alphas:
+        por, solid, q = O.smooth_sphere_geometry(nx, ny, nz, (0.0, 0.0, 0.0), torch.tensor(0.5, dtype=dt), a)
+        u = torch.zeros((3, nx, ny, nz), dtype=dt)
+        u[0] = u_in
+        f = O.equilibrium(torch.ones((nx, ny, nz), dtype=dt), u)
+        vel = torch.tensor([u_in, 0.0, 0.0], dtype=dt)
+        area = torch.clamp(O.projected_area(solid, 0), min=1.0)
+        t0 = time.perf_counter()
+        with torch.no_grad():
+            fp = f
+            for _ in range(n_steps):
+                fc = O.soft_bounce_back(O.mrt(f, tau), por)
+                fp = fc
+                f = O.zou_he_outlet(O.zou_he_inlet(O.stream(fc, q, True), vel), 1.0)
+        cd = float(O.drag_coefficient(O.momentum_exchange(fp, f, q), u_in, area))
+        out["cd"][str(a)] = cd
+        print(f"ex12 alpha {a}: Cd {cd:.9f} ({time.perf_counter() - t0:.0f} s)", flush=True)
+    _write("ex12_sharpness_oracle_f32.json", out)
+
+
 if __name__ == "__main__":
     which = sys.argv[1:] or ["kats"]
     for w in which:
         if w.startswith("c2:"):
             c2(tuple(w[3:].split(",")))  # e.g. c2:f32 adds the float32 series
+        elif w.startswith("c2p:"):
+            c2p(tuple(int(v) for v in w[4:].split(",")))  # e.g. c2p:4,5 adds those seeds
         else:
-            {"kats": kats, "c1": c1, "c2": c2, "c3": c3}[w]()
+            {"kats": kats, "c1": c1, "c2": c2, "c3": c3, "c2p": c2p, "ex12": ex12}[w]()

@@ -1,8 +1,9 @@
 """Properties that hold at any size, checked at the 256x128x128 benchmark size
 (the oracle is too slow there): the fused kernel's per-step force equals the
 independent per-operator momentum exchange, state layouts are bit-identical,
-float32 tracks float64, checkpointed and store-all gradients are
-bit-identical, and the halfway hard mask gives an exactly zero dCd/dr."""
+float32 tracks float64 (states, forces and the adjoint's dCd/dr),
+checkpointed and store-all gradients are bit-identical, and the halfway hard
+mask gives an exactly zero dCd/dr."""
 
 import pytest
 import torch
@@ -65,18 +66,19 @@ def test_float32_tracks_float64(dev):
     assert float((F32 - F64).abs().max()) < 1e-6 * float(F64.abs().max()) + 1e-9
 
 
-def _grad(dev, steps, checkpoint, halfway=False):
+def _grad(dev, steps, checkpoint, halfway=False, storage="f32"):
+    dt = torch.float64 if storage == "f64" else torch.float32
     r = torch.tensor(0.5, dtype=torch.float32, device=dev, requires_grad=True)
-    _, solid, q = lbm.smooth_sphere_geometry(NX, NY, NZ, (0.0, 0.0, 0.0), r, 20.0, device=dev)
+    _, solid, q = lbm.smooth_sphere_geometry(NX, NY, NZ, (0.0, 0.0, 0.0), r, 20.0, device=dev, out_dtype=dt)
     porosity = None
     if halfway:
         # q = 1/2 on every link and the reference's 0 * porosity(r) term
         por, _, _ = lbm.smooth_sphere_geometry(NX, NY, NZ, (0.0, 0.0, 0.0), r, 20.0, device=dev)
         q = lbm.compute_q_from_solid(solid, NX, NY, NZ, 0.5)
         porosity, solid = solid.to(torch.float32) + 0.0 * por, None
-    _, _, F = A.rollout(_f0(dev), steps, tau=TAU, q=q, solid=solid, porosity=porosity,
+    _, _, F = A.rollout(_f0(dev, dt), steps, tau=TAU, q=q, solid=solid, porosity=porosity,
                         inlet_velocity=(U, 0.0, 0.0), collision="mrt", guard="pre", guard_velocity=(U, 0.0, 0.0),
-                        offset="inlet", checkpoint=checkpoint)
+                        offset="inlet", checkpoint=checkpoint, storage=storage)
     area = torch.clamp(lbm.projected_area(lbm.create_sphere(NX, NY, NZ, (0.0, 0.0, 0.0), 0.5, device=dev)[0])
                        .to(torch.float64), min=1.0)
     loss = A.drag_series(F, U, area)[-max(1, steps // 3):].mean()
@@ -92,3 +94,13 @@ def test_checkpointed_gradient_bit_identical_to_store_all(dev):
 
 def test_halfway_hard_gradient_exactly_zero(dev):
     assert float(_grad(dev, 40, "all", halfway=True)) == 0.0
+
+
+def test_float32_gradient_matches_float64(dev):
+    """The float32 adjoint sweep of grids of 2^21 cells or more runs in
+    512-thread workgroups, the float64 sweep in 256-thread ones: an
+    independent check of the 512-thread launch (40 steps, store-all)."""
+    g32 = float(_grad(dev, 40, "all"))
+    g64 = float(_grad(dev, 40, "all", storage="f64"))
+    print(f"256x128x128 40 steps: dCd/dr f32 {g32:.10f} f64 {g64:.10f} rel {abs(g32 / g64 - 1.0):.2e}")
+    assert abs(g32 / g64 - 1.0) < 1e-4

@@ -153,6 +153,42 @@ def test_momentum_exchange_decomposed_invariants(dev, dt):
     assert abs(float(tot[0]) - float((P[1, 7] + S[2, 7]).sum())) < 100 * TOL[dt]
 
 
+def test_wrong_shapes_raise(dev):
+    """A short solid/q/porosity raises before a kernel indexes it by the grid."""
+    from lbm_jax_autodiff_amd.engine import Lattice
+    f = near_eq_state(NX, NY, NZ).to(dev)
+    short = torch.zeros((NX, NY, NZ - 1), dtype=torch.int32, device=dev)
+    with pytest.raises(ValueError, match="shape"):
+        lbm.core.hard_bounce_back(f, short)
+    with pytest.raises(ValueError, match="shape"):
+        Lattice(NX, NY, NZ, tau=0.6, solid=short, device=dev)
+    with pytest.raises(ValueError, match="shape"):
+        Lattice(NX, NY, NZ, tau=0.6, porosity=short.float(), device=dev)
+    with pytest.raises(ValueError, match="shape"):
+        lbm.stream(f, -torch.ones((19, NX, NY, NZ - 1), dtype=f.dtype, device=dev))
+    with pytest.raises(ValueError, match="shape"):
+        lbm.soft_bounce_back(f, short.to(f.dtype))
+
+
+def test_stability_guard_operator(dev, dt):
+    """Per-operator guard (state.py:158-169) against the oracle's post guard;
+    its VJP passes nothing at reset cells."""
+    f = near_eq_state(NX, NY, NZ)
+    f[:, 1, 2, 3] *= 3.0          # rho > 2
+    f[1, 4, 1, 1] += 0.9          # |u| > 0.4
+    f[:, 5, 0, 0] *= 0.2          # rho < 0.3
+    bad = O._guard_mask(f)
+    assert int(bad.sum()) == 3
+    ref = torch.where(bad.unsqueeze(0), O.equilibrium(torch.ones_like(bad, dtype=torch.float64),
+                                                      torch.zeros((3,) + bad.shape, dtype=torch.float64)), f)
+    fd = f.to(dt).to(dev).requires_grad_(True)
+    out = lbm.core.stability_guard(fd)
+    assert maxabs(out, ref) < TOL[dt]
+    (g,) = torch.autograd.grad(out.sum(), fd)
+    expect = (~bad).to(torch.float64).unsqueeze(0).expand(19, -1, -1, -1)
+    assert maxabs(g, expect) == 0.0
+
+
 def test_projected_area_and_cd(dev):
     solid = torch.zeros((16, 16, 16), dtype=torch.int32)
     solid[6:10, 6:10, 6:10] = 1  # box 4x4 in the y-z shadow (test_forces.py:62-67)
