@@ -290,6 +290,21 @@ int lbm_step_chain(const lbm_step_desc* d, int n, void* const* states, double* f
 int lbm_force_reduce(const double* partials, int T, int slots, double* force,
                      lbm_stream_t stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): lbm_step_chain followed by lbm_force_reduce in one call, n full
+ * steps states[k] -> states[k+1] (host array of n+1 device pointers) and the
+ * per-step momentum-exchange force force[n][3] (float64), with the
+ * per-workgroup force partials held in a library-owned device array of
+ * LBM_NOALLOC_PARTIALS doubles per GPU.  Allocates nothing (the array is a
+ * file-scope device variable; its address is looked up once per device).
+ * Returns LBM_E_ARG when n * lbm_step_force_slots(d) * 3 exceeds
+ * LBM_NOALLOC_PARTIALS.  Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (different streams, or host threads) race. */
+#define LBM_NOALLOC_PARTIALS (1u << 20)
+int lbm_step_chain_forces_noalloc(const lbm_step_desc* d, int n, void* const* states,
+                                  double* force, lbm_stream_t stream);
+
 /* ---- Adjoint of the fused timestep --------------------------------------
  * The reference differentiates its rollout with jax.grad / value_and_grad
  * (examples/11_optimisation_v2.py:176, tests/test_gradients_f64.py:129-217);

@@ -19,6 +19,7 @@ LBM_COLL_BGK, LBM_COLL_MRT = 0, 1
 LBM_BOUNCE_NONE, LBM_BOUNCE_HARD, LBM_BOUNCE_SOFT = 0, 1, 2
 LBM_GUARD_NONE, LBM_GUARD_POST, LBM_GUARD_PRE = 0, 1, 2
 LBM_MODE_FULL, LBM_MODE_COLLIDE, LBM_MODE_STREAM = 0, 1, 2
+LBM_NOALLOC_PARTIALS = 1 << 20  # doubles of the no-allocation entry point's partials array
 
 _ERRORS = {1001: "LBM_E_ARG (bad size, pointer or flag)", 1002: "LBM_E_DTYPE (dtype not supported)",
            1003: "LBM_E_LAUNCH (kernel launch failed)"}
@@ -88,6 +89,7 @@ _SIGS = {
                                   c_void_p, c_void_p]),
     "lbm_step": (c_int, [ctypes.POINTER(StepDesc), c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "lbm_force_reduce": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
+    "lbm_step_chain_forces_noalloc": (c_int, [ctypes.POINTER(StepDesc), c_int, c_void_p, c_void_p, c_void_p]),
     "lbm_convert_state": (c_int, [ctypes.POINTER(StepDesc), c_int, c_void_p, c_int, c_void_p, c_void_p]),
     "lbm_step_adjoint": (c_int, [ctypes.POINTER(StepDesc), ctypes.c_void_p, c_int, c_void_p, c_void_p, c_void_p,
                                  c_void_p, c_void_p]),

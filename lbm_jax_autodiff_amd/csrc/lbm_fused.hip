@@ -22,6 +22,8 @@
 // populations the sweep is bound by vector instructions rather than memory,
 // and where nx is a multiple of 64 it takes each wave's row addresses from
 // scalar registers (dense_row).
+#include <mutex>
+
 #include "lbm_physics.hpp"
 
 namespace lbm {
@@ -673,6 +675,36 @@ int launch_step(const lbm_step_desc* d, int mode, const void* src, void* dst, do
   return LBM_E_ARG;
 }
 
+// Force partials of lbm_step_chain_forces_noalloc: a file-scope device array
+// (one copy per GPU, alive until the process ends) so that entry point
+// allocates nothing.  Host code never passes `scratch` itself to a kernel:
+// in host code the name is a stand-in, not a device address; the address of
+// the current device's copy comes from hipGetSymbolAddress (noalloc_scratch).
+constexpr size_t kNoAllocPartials = size_t(1) << 20;  // doubles (8 MiB)
+static __device__ __align__(256) double scratch[kNoAllocPartials];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mutex;
+static double* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current device's `scratch`, looked up on the first
+// call on each device (with that device current) and kept.
+inline int noalloc_scratch(double** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return int(e);
+  if (dev < 0 || dev >= kMaxDevices) return LBM_E_ARG;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return int(e);
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return LBM_OK;
+}
+
 }  // namespace lbm
 
 using namespace lbm;
@@ -724,6 +756,23 @@ int lbm_force_reduce(const double* partials, int T, int slots, double* force, lb
   if (T <= 0 || !force || slots < 0 || (slots > 0 && !partials)) return LBM_E_ARG;
   hipLaunchKernelGGL(k_force_reduce, (T + 255) / 256, 256, 0, (hipStream_t)stream, partials, T, slots, force);
   return hipGetLastError() == hipSuccess ? LBM_OK : LBM_E_LAUNCH;
+}
+
+int lbm_step_chain_forces_noalloc(const lbm_step_desc* d, int n, void* const* states, double* force,
+                                  lbm_stream_t stream) {
+  if (!d || n <= 0 || !states || !force) return LBM_E_ARG;
+  const int slots = lbm_step_force_slots(d);
+  if (slots > 0 && size_t(n) * size_t(slots) * 3u > kNoAllocPartials) return LBM_E_ARG;
+  double* part = nullptr;
+  if (slots > 0) {
+    const int rc = noalloc_scratch(&part);
+    if (rc) return rc;
+  }
+  // every link workgroup writes its slot of every step, so the partials need
+  // no clearing; without links the reduction writes zero forces
+  int rc = lbm_step_chain(d, n, states, part, int64_t(slots) * 3, stream);
+  if (rc) return rc;
+  return lbm_force_reduce(part, n, slots, force, stream);
 }
 
 int lbm_convert_state(const lbm_step_desc* d, int src_dtype, const void* src, int dst_dtype, void* dst,

@@ -282,6 +282,24 @@ class Lattice:
         L.call("lbm_step_chain", ctypes.byref(self.desc), n, L.ptr_array(states), fp, stride,
                L.stream_handle(self.device))
 
+    def chain_forces_noalloc(self, states, force: torch.Tensor):
+        """len(states) - 1 full steps and their forces (float64, (n, 3)) in one
+        call that allocates no device memory: the force partials live in the
+        library's own device array (lbm_step_chain_forces_noalloc), which
+        bounds n * slots * 3 by LBM_NOALLOC_PARTIALS and is shared by every
+        call on the device."""
+        n = len(states) - 1
+        if n <= 0:
+            raise ValueError("need at least two states")
+        if n * self.slots * 3 > L.LBM_NOALLOC_PARTIALS:
+            raise ValueError(f"{n} steps x {self.slots} force slots exceed the no-allocation partials array "
+                             f"({L.LBM_NOALLOC_PARTIALS} doubles)")
+        F.require_shape(force, "force", (n, 3), self.device)
+        if force.dtype != torch.float64 or not force.is_contiguous():
+            raise ValueError("force must be a contiguous float64 tensor")
+        L.call("lbm_step_chain_forces_noalloc", ctypes.byref(self.desc), n, L.ptr_array(states), L.ptr(force),
+               L.stream_handle(self.device))
+
     def pingpong(self, cur, spare_a, spare_b, n, fpart=None):
         """n full steps from cur through two spare buffers (never writing cur
         unless it is one of them); returns the buffer holding the result."""
