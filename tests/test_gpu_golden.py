@@ -9,13 +9,20 @@ outputs of the CPU oracle at full length:
 
 Tolerances: float64 kernels agree with the float64 oracle to rounding.  The
 float32 kernels meet the bars of the workload table for the profile (1e-5
-absolute) and dCd/dr (1e-3 relative).  The 64x32x32 sphere started from rest
-rings (Cd swings between 0 and 22 in the first 2000 steps), and float32
-rounding noise moves its time-averaged Cd: float32 runs from initial states
-perturbed by 1e-8 land between -7e-5 and +2.3e-4 relative of the float64
-result (the oracle's float32 run: +8e-6), while the same perturbations move
-the float64 result by under 2e-6.  The float32 Cd is therefore held to 5e-4
-against float64 and against the float32 oracle.
+absolute) and dCd/dr (1e-3 relative).
+
+The 64x32x32 sphere starts from rest in a state symmetric in y and z (uniform
+populations, the sphere on the centre line), and its Cd swings between 0 and
+22 in the first 2000 steps.  Per step, float32 leaves float64 by more than
+1e-4 within the first 15-45 steps, in the kernels and in the oracle alike, so
+no early window holds a 1e-4 bar.  Unperturbed, the wake's symmetry is
+broken by rounding alone, and the tail mean of that one run is +2.3e-4
+(kernels) or +8e-6 (oracle) from float64: it is held to 5e-4.  From starts
+perturbed by 1e-8 (tests/_helpers.py perturbed_start) the float32 kernels'
+tail mean is -6.1e-5 +- 2.1e-5 from float64 over 32 seeds (one at -1.03e-4),
+the float32 oracle's +3.0e-5 +- 3.1e-5 over 11 (tests/golden/c2_sphere_cd_
+64x32x32_perturbed_f32.json), and the float64 kernels' -2e-7 +- 1e-6: the
+1e-4 bar is pinned on the mean over eight perturbed starts, each within 2e-4.
 """
 
 import json
@@ -77,6 +84,31 @@ def test_c2_time_averaged_cd(dev, storage, tol):
         ref_series = torch.tensor(g["f64"]["cd"], dtype=torch.float64)
         got = cd.detach().cpu().to(torch.float64)
         assert float(((got - ref_series).abs() / ref_series.abs().clamp(min=1e-3)).max()) <= 1e-8
+
+
+@pytest.mark.parametrize("storage", ["f32", "f64"])
+def test_c2_time_averaged_cd_perturbed_ensemble(dev, storage):
+    """Tail-mean Cd from eight initial states perturbed by 1e-8 (seeds 1-8):
+    float32 mean within 1e-4 of the float64 oracle and each run within 2e-4;
+    float64 each within 5e-6."""
+    from _helpers import perturbed_start
+    from lbm_jax_autodiff_amd.adjoint import drag_series
+    g = _load("c2_sphere_cd_64x32x32.json")
+    ref = _tail_mean(g["f64"]["cd"])
+    nx, ny, nz = 64, 32, 32
+    lat, _, _, area = P.sphere_lattice(nx, ny, nz, P.world_center(nx, ny, nz, (16, 16, 16)), 0.5, storage=storage)
+    dt = torch.float64 if storage == "f64" else torch.float32
+    rels = []
+    for seed in range(1, 9):
+        _, _, F = lat.run(perturbed_start(nx, ny, nz, seed).to(dt).to(dev), g["steps"])
+        cd = drag_series(F, 0.05, area).detach().cpu().to(torch.float64).tolist()
+        rels.append(_tail_mean(cd) / ref - 1.0)
+    mean = sum(rels) / len(rels)
+    print(f"c2 {storage} perturbed: mean rel {mean:.3e}, runs {[f'{r:.2e}' for r in rels]}")
+    if storage == "f32":
+        assert abs(mean) <= 1e-4 and max(abs(r) for r in rels) <= 2e-4
+    else:
+        assert max(abs(r) for r in rels) <= 5e-6
 
 
 @pytest.mark.parametrize("storage,tol", [("f64", 1e-8), ("f32", 1e-3)])
