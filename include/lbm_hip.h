@@ -314,7 +314,9 @@ int lbm_step_chain_forces_noalloc(const lbm_step_desc* d, int n, void* const* st
 
 typedef struct lbm_adjoint_desc {
   double* dq_acc;          /* 18 x n_links: += dL/dq of each link (may be null) */
-  void* mu_link;           /* 19 x n_links scratch: per-step link-cell cotangents */
+  void* mu_link;           /* 19 x n_links scratch: per-step link-cell cotangents
+                              (with fold_links: their additions to the link
+                              cells' own entries) */
   double* phi_acc;         /* N: += dL/dporosity (soft bounce; may be null) */
   double* u_acc;           /* 3 x (ny*nz): += dL/du_in per inlet cell (may be null) */
   int32_t n_targets;       /* size of the scatter plan */
@@ -326,6 +328,15 @@ typedef struct lbm_adjoint_desc {
                               double for LBM_F64); LBM_F16DEV = IEEE binary16
                               holding lambda * lam_scale (float arithmetic only) */
   double lam_scale;        /* power of two for LBM_F16DEV (<= 0 means 1) */
+  void* mu_link_alt;       /* second 19 x n_links buffer for fold_links (a chain
+                              alternates mu_link and mu_link_alt) */
+  int32_t fold_links;      /* 0: the link additions come from lbm_adjoint_scatter
+                              and the plan above, one launch per step;
+                              1: no plan -- lbm_step_adjoint writes the Bouzidi
+                              second-neighbour terms with its owner writes and
+                              lbm_adjoint_link_finish adds the link cells' own
+                              terms; lbm_adjoint_chain folds the latter into the
+                              next step's loads and finishes its last step */
 } lbm_adjoint_desc;
 
 /* One adjoint step.  mode LBM_MODE_FULL: state = P_t,
@@ -334,8 +345,9 @@ typedef struct lbm_adjoint_desc {
  * LBM_MODE_COLLIDE: state = S_0, lam_in = dL/dP_0 -> lam_out = dL/dS_0 (in the
  * arithmetic type, whatever lam_storage is).
  * gF (3 doubles on the device, may be null) = dL/dF_t of this step's
- * momentum-exchange force.  Requires periodic_yz and nx >= 2.  Must be
- * followed by lbm_adjoint_scatter for modes FULL and STREAM. */
+ * momentum-exchange force.  Requires periodic_yz and nx >= 2.  Modes FULL and
+ * STREAM must be followed by lbm_adjoint_scatter, or with a->fold_links by
+ * lbm_adjoint_link_finish. */
 int lbm_step_adjoint(const lbm_step_desc* d, const lbm_adjoint_desc* a, int mode,
                      const void* state, const void* lam_in, void* lam_out,
                      const double* gF, lbm_stream_t stream);
@@ -344,9 +356,20 @@ int lbm_step_adjoint(const lbm_step_desc* d, const lbm_adjoint_desc* a, int mode
 int lbm_adjoint_scatter(const lbm_step_desc* d, const lbm_adjoint_desc* a, void* lam,
                         const double* gF, lbm_stream_t stream);
 
+/* a->fold_links = 1: completes a lbm_step_adjoint as lbm_adjoint_scatter does,
+ * without a plan: adds to lam the Bouzidi and force cotangents of the link
+ * cells' own populations (streaming.py:155-178, forces.py:95-118), which that
+ * step stored per link cell in mu (19 x n_links; null = a->mu_link).  One
+ * thread per link cell. */
+int lbm_adjoint_link_finish(const lbm_step_desc* d, const lbm_adjoint_desc* a, const void* mu,
+                            void* lam, lbm_stream_t stream);
+
 /* Adjoint of n consecutive FULL steps: for k = n-1 down to 0,
  * lbm_step_adjoint(FULL, states[k]) followed by lbm_adjoint_scatter, with
- * gF + 3k the force cotangent of step k.  On entry lam_a holds the
+ * gF + 3k the force cotangent of step k.  With a->fold_links, one launch per
+ * step: each launch adds the previous one's link-cell additions to the
+ * cotangents it loads (mu_link and mu_link_alt alternate), and the chain
+ * ends with lbm_adjoint_link_finish.  On entry lam_a holds the
  * cotangent of the state after the last step; the two buffers alternate, so
  * the result (the cotangent of states[0]) is in lam_a for even n and in
  * lam_b for odd n.  The transpose of the reference's scan loop under

@@ -94,6 +94,7 @@ _SIGS = {
     "lbm_step_adjoint": (c_int, [ctypes.POINTER(StepDesc), ctypes.c_void_p, c_int, c_void_p, c_void_p, c_void_p,
                                  c_void_p, c_void_p]),
     "lbm_adjoint_scatter": (c_int, [ctypes.POINTER(StepDesc), ctypes.c_void_p, c_void_p, c_void_p, c_void_p]),
+    "lbm_adjoint_link_finish": (c_int, [ctypes.POINTER(StepDesc), ctypes.c_void_p, c_void_p, c_void_p, c_void_p]),
     "lbm_equilibrium_vjp": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,
                                     c_void_p]),
     "lbm_density_vjp": (c_int, [c_int, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),

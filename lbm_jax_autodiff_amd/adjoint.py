@@ -43,6 +43,7 @@ class AdjointDesc(ctypes.Structure):
         ("u_acc", ctypes.c_void_p), ("n_targets", ctypes.c_int32), ("tgt_index", ctypes.c_void_p),
         ("tgt_start", ctypes.c_void_p), ("src", ctypes.c_void_p), ("coef", ctypes.c_void_p),
         ("lam_storage", ctypes.c_int32), ("lam_scale", ctypes.c_double),
+        ("mu_link_alt", ctypes.c_void_p), ("fold_links", ctypes.c_int32),
     ]
 
 
@@ -253,9 +254,25 @@ class _RolloutFn(torch.autograd.Function):
 
         ad = AdjointDesc()
         keep = []
+        # link-cell cotangents of two consecutive steps: the chain folds each
+        # step's link additions into the next step's loads (fold_links), so no
+        # scatter plan and no extra launch per step
         mu = torch.zeros((19, max(n, 1)), dtype=real, device=dev)
         keep.append(mu)
         ad.mu_link = L.ptr(mu)
+        fold = ctx.cfg["link_additions"] == "fold"
+        if fold:
+            mu_alt = torch.zeros_like(mu)
+            keep.append(mu_alt)
+            ad.mu_link_alt = L.ptr(mu_alt)
+            ad.fold_links = 1
+        else:
+            # one scatter launch per step from a per-entry plan
+            plan = scatter_plan(lat)
+            if plan is not None:
+                keep.extend(plan)
+                ad.n_targets = int(plan[0].numel())
+                ad.tgt_index, ad.tgt_start, ad.src, ad.coef = (L.ptr(t) for t in plan)
         dq = None
         if want_q:
             dq = torch.zeros((18, n), dtype=torch.float64, device=dev)
@@ -268,11 +285,6 @@ class _RolloutFn(torch.autograd.Function):
         if want_u:
             u_acc = torch.zeros((ny * nz, 3), dtype=torch.float64, device=dev)
             ad.u_acc = L.ptr(u_acc)
-        plan = scatter_plan(lat)
-        if plan is not None:
-            keep.extend(plan)
-            ad.n_targets = int(plan[0].numel())
-            ad.tgt_index, ad.tgt_start, ad.src, ad.coef = (L.ptr(t) for t in plan)
         if gF is None:
             gF = torch.zeros((n_steps, 3), dtype=torch.float64, device=dev)
         gF = gF.detach().to(torch.float64).contiguous()
@@ -308,7 +320,10 @@ class _RolloutFn(torch.autograd.Function):
                 assert t1 == n_steps
                 L.call("lbm_step_adjoint", dptr, aptr, L.LBM_MODE_STREAM, L.ptr(have[t]), L.ptr(g_in), L.ptr(cur),
                        L.ptr(gF[t]), stream)
-                L.call("lbm_adjoint_scatter", dptr, aptr, L.ptr(cur), L.ptr(gF[t]), stream)
+                if fold:
+                    L.call("lbm_adjoint_link_finish", dptr, aptr, L.ptr(mu), L.ptr(cur), stream)
+                else:
+                    L.call("lbm_adjoint_scatter", dptr, aptr, L.ptr(cur), L.ptr(gF[t]), stream)
                 if gP is not None:
                     cur.copy_(cur.to(real) + gP.detach().permute(0, 3, 2, 1).to(real) * scale)
                 t1 -= 1
@@ -347,7 +362,7 @@ def rollout(f0: torch.Tensor, n_steps: int, *, tau, q: Optional[torch.Tensor] = 
             collision: str = "bgk", smagorinsky_cs: Optional[float] = None, guard: str = "none",
             guard_velocity=None, storage: str = "f32", offset="weights",
             checkpoint: Union[str, int, Sequence[int]] = "all", adjoint_storage: Optional[str] = None,
-            layout: str = "rows", cells_per_thread: int = 0):
+            layout: str = "rows", cells_per_thread: int = 0, link_additions: str = "fold"):
     """Differentiable fused rollout: (S_N, P_{N-1}, forces[N, 3] float64).
 
     Step order as in the reference's drivers: [pre guard] -> [Smagorinsky]
@@ -358,6 +373,10 @@ def rollout(f0: torch.Tensor, n_steps: int, *, tau, q: Optional[torch.Tensor] = 
     them), halving the adjoint's cotangent traffic.  ``layout`` places the
     state and cotangent buffers ("rows" or "apart", as for ``Lattice``);
     ``cells_per_thread`` selects the sweeps' addressing as for ``Lattice``.
+    ``link_additions`` "fold" (default) lets each adjoint step add the
+    previous step's Bouzidi and force cotangents as it loads them (one launch
+    per step); "scatter" adds them with a separate launch per step from a
+    per-entry plan (the same sums in another order).
     """
     nx, ny, nz = F.dims_of_pop(f0)
     if isinstance(inlet_velocity, torch.Tensor):
@@ -371,7 +390,10 @@ def rollout(f0: torch.Tensor, n_steps: int, *, tau, q: Optional[torch.Tensor] = 
     cfg = dict(n_steps=int(n_steps), nx=nx, ny=ny, nz=nz, tau=F.scalar_float(tau), rho_out=float(rho_out),
                collision=collision, smagorinsky_cs=smagorinsky_cs, solid=solid, guard=guard,
                guard_velocity=guard_velocity, storage=storage, offset=offset, checkpoint=checkpoint,
-               adjoint_storage=adjoint_storage, layout=layout, cells_per_thread=int(cells_per_thread))
+               adjoint_storage=adjoint_storage, layout=layout, cells_per_thread=int(cells_per_thread),
+               link_additions=link_additions)
+    if link_additions not in ("fold", "scatter"):
+        raise ValueError("link_additions must be 'fold' or 'scatter'")
     if int(n_steps) < 1:
         raise ValueError("n_steps must be >= 1")
     return _RolloutFn.apply(f0, q, porosity, u, cfg)

@@ -43,6 +43,13 @@ struct AdjArgs {
   R* mu_link;
   double* phi_acc;
   double* u_acc;
+  // folded link additions (lbm_adjoint_desc.fold_links): with `fold` the
+  // link cells write their Bouzidi second-neighbour terms with their owner
+  // writes and store, in mu_link, the additions to their own 19 entries
+  // instead of their cotangents; add_prev (the previous launch's such
+  // additions, null in a chain's first launch) is added to what they load
+  int fold;
+  const R* add_prev;
   // cotangent storage scale (half storage) and its inverse
   R lam_scale, lam_inv;
   // extents in bytes of a state and of a cotangent buffer (below 2^32 when
@@ -59,6 +66,8 @@ struct LamPlain {
   using T = R;
   __device__ static R ld(const T* p, size_t k, R) { return p[k]; }
   __device__ static void st(T* p, size_t k, R v, R) { p[k] = v; }
+  // the value a store and a load would give back
+  __device__ static R rt(R v, R, R) { return v; }
   __device__ static R bld(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, R) {
     if constexpr (sizeof(R) == 4) return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, int(vo), int(so), 0));
     else return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b64(r, int(vo), int(so), 0));
@@ -72,6 +81,9 @@ struct LamHalf {
   using T = unsigned short;
   __device__ static float ld(const T* p, size_t k, float inv_scale) { return f16_bits_to_float(p[k]) * inv_scale; }
   __device__ static void st(T* p, size_t k, float v, float scale) { p[k] = float_to_f16_bits(v * scale); }
+  __device__ static float rt(float v, float scale, float inv_scale) {
+    return f16_bits_to_float(float_to_f16_bits(v * scale)) * inv_scale;
+  }
   __device__ static float bld(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, float inv_scale) {
     return f16_bits_to_float(__builtin_amdgcn_raw_buffer_load_b16(r, int(vo), int(so), 0)) * inv_scale;
   }
@@ -193,6 +205,34 @@ __device__ __forceinline__ void pull_link(const AdjArgs<typename S::R>& a, const
   });
 }
 
+// The cotangent addition a link cell's Bouzidi pulls and force make to its
+// own entry P_k(x) (streaming.py:155-178, forces.py:95-118), from the
+// cotangents mu of its pulled values and the step's force seed g:
+//   a wall in direction opp(k) (q_k >= 0): c mu_opp(k) + e_k . g, with
+//     c = 1/(2 q_k) (q_k >= 1/2) or 2 q_k (below);
+//   a wall in direction k on the high branch (q_opp(k) >= 1/2):
+//     (1 - 1/(2 q_opp(k))) mu_k.
+// The remaining term, (1 - 2q) mu_i into P_opp(i)(x + e_i), goes to an entry
+// the cell owner-writes itself.  The coefficients are the forward's
+// (storage-precision q).
+template <int K, typename R>
+__device__ __forceinline__ R own_link_addition(const R* q, const R* mu, R gx, R gy, R gz) {
+  constexpr int ko = opp(K);
+  R add = R(0);
+  const R qk = q[K];
+  if (qk >= R(0)) {
+    const R c = qk >= R(0.5) ? R(1) / (R(2) * fmin(qk, R(1))) : R(2) * qk;
+    R f = R(0);
+    add_scaled<EX[K]>(f, gx);
+    add_scaled<EY[K]>(f, gy);
+    add_scaled<EZ[K]>(f, gz);
+    add = fma(c, mu[ko], f);
+  }
+  const R qo = q[ko];
+  if (qo >= R(0.5)) add = fma(R(1) - R(1) / (R(2) * fmin(qo, R(1))), mu[K], add);
+  return add;
+}
+
 template <typename R>
 __device__ __forceinline__ void apply_zou_he(const AdjArgs<R>& a, R* s, int x) {
   if (x == 0) zou_he_in(s, a.uin[0], a.uin[1], a.uin[2], false, R(0));
@@ -234,8 +274,16 @@ __device__ __forceinline__ bool collide_transpose(const AdjArgs<R>& a, R* s, R* 
       g[i] = R(0);
     });
   } else {
-    const R gom = (COLL == LBM_COLL_MRT) ? mrt_vjp(ps, m, g, omega) : bgk_vjp(ps, m, g, omega);
-    if (a.smag) smag_vjp(ps, m, g, a.tau, a.cs, -gom * omega * omega);
+    // dL/domega is needed only for the Smagorinsky path (a uniform branch;
+    // without it the transpose skips the stress moments' dm)
+    if (a.smag) {
+      const R gom = (COLL == LBM_COLL_MRT) ? mrt_vjp<true>(ps, m, g, omega) : bgk_vjp<true>(ps, m, g, omega);
+      smag_vjp(ps, m, g, a.tau, a.cs, -gom * omega * omega);
+    } else if constexpr (COLL == LBM_COLL_MRT) {
+      mrt_vjp<false>(ps, m, g, omega);
+    } else {
+      bgk_vjp<false>(ps, m, g, omega);
+    }
   }
   return pre_reset;
 }
@@ -325,6 +373,9 @@ __device__ __forceinline__ void adjoint_cell(const AdjArgs<typename S::R>& a, co
       constexpr int j = J;
       if constexpr (j != 0) lmask |= (qv[j] >= R(0)) ? (1u << j) : 0u;
     });
+    // qv lives in LDS: reload it for the owner writes instead of holding 18
+    // registers through the transposed collision
+    asm volatile("" ::: "memory");
   } else {
     pull_dense<S>(a, P, o, x, y, z, s);
   }
@@ -334,6 +385,17 @@ __device__ __forceinline__ void adjoint_cell(const AdjArgs<typename S::R>& a, co
       constexpr int i = I;
       gg[i] = LM::ld(lam_in, i * N + o, a.lam_inv);
     });
+    if constexpr (LINK) {
+      if (a.add_prev) {
+        // the previous launch's additions to this cell's own entries, rounded
+        // to the cotangent storage as lbm_adjoint_link_finish would store them
+        sfor<Q>([&](auto I) {
+          constexpr int i = I;
+          if constexpr (i != 0)
+            gg[i] = LM::rt(gg[i] + a.add_prev[size_t(i) * a.n_links + b], a.lam_scale, a.lam_inv);
+        });
+      }
+    }
   };
   cell_transpose<S, COLL, MODE, LINK>(a, s, g, R(1), load_g, c, x, fl, skip, lmask, gF);
   // owner writes and Bouzidi bookkeeping
@@ -345,14 +407,37 @@ __device__ __forceinline__ void adjoint_cell(const AdjArgs<typename S::R>& a, co
       constexpr int io = opp(i);
       const bool link = LINK && (lmask & (1u << io));
       const bool normal = !link && !x_fallback<i>(x, a.nx);
-      LM::st(lam_out, i * N + owner_target<i>(a, x, y, z), normal ? g[i] : R(0), a.lam_scale);
+      R v = normal ? g[i] : R(0);
       if constexpr (LINK) {
-        a.mu_link[size_t(i) * a.n_links + b] = link ? g[i] : R(0);
+        // folded: entry (x - e_i, i) is also the low-branch second neighbour
+        // of the wall direction opp(i) (q_i in [0, 1/2)): (1 - 2 q_i) mu_opp(i)
+        if (a.fold && qv[i] >= R(0) && qv[i] < R(0.5)) v = fma(R(1) - R(2) * qv[i], g[io], v);
+      }
+      LM::st(lam_out, i * N + owner_target<i>(a, x, y, z), v, a.lam_scale);
+      if constexpr (LINK) {
+        if (!a.fold) a.mu_link[size_t(i) * a.n_links + b] = link ? g[i] : R(0);
         // d S_i / d q_opp(i) on the taken branch (from the pull)
         if (link && a.dq_acc) a.dq_acc[size_t(io - 1) * a.n_links + b] += double(g[i]) * double(dsdq[i]);
       }
     }
   });
+  if constexpr (LINK) {
+    if (a.fold) {
+      // this step's additions to the cell's own entries, for the next launch
+      // (or lbm_adjoint_link_finish) to add to what it loads; one entry at a
+      // time, stored at once, so the link path keeps its register budget
+      R gx = R(0), gy = R(0), gz = R(0);
+      if (gF) {
+        gx = R(gF[0]);
+        gy = R(gF[1]);
+        gz = R(gF[2]);
+      }
+      sfor<Q>([&](auto K) {
+        constexpr int k = K;
+        if constexpr (k != 0) a.mu_link[size_t(k) * a.n_links + b] = own_link_addition<k>(qv, g, gx, gy, gz);
+      });
+    }
+  }
 }
 
 // Dense cell when every wave's 64 cells share one (y, z) row (nx a multiple
@@ -436,21 +521,19 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(adj_waves_pe
   using R = typename S::R;
   const uint32_t bid = blockIdx.x;
   if (int(bid) < a.nlb) {
-    // per-thread dS/dq of the link directions, kept out of the registers
-    __shared__ R dsdq[BS][Q];
-    const int b = int(bid) * BS + int(threadIdx.x);
-    if (b >= a.n_links) return;
+    // per-thread q and dS/dq of the link directions, kept out of the
+    // registers while the collision is transposed
+    __shared__ R qsh[kLinkCellsPerBlock][Q];
+    __shared__ R dsdq[kLinkCellsPerBlock][Q];
+    const int b = int(bid) * kLinkCellsPerBlock + int(threadIdx.x);
+    if (int(threadIdx.x) >= kLinkCellsPerBlock || b >= a.n_links) return;
     const uint32_t c = uint32_t(a.link_cell[b]);
-    R qv[Q];
+    R* qv = qsh[threadIdx.x];
     qv[0] = R(-1);
     sfor<Q>([&](auto I) {
       constexpr int i = I;
       if constexpr (i != 0) qv[i] = R(a.link_q[size_t(i - 1) * a.n_links + b]);
     });
-    // pin the converted q here (a conversion sunk to its uses keeps the
-    // float64 values alive)
-#pragma unroll
-    for (int i = 1; i < Q; ++i) asm volatile("" : "+v"(qv[i]));
     adjoint_cell<S, LM, COLL, MODE, true>(a, P, lam_in, lam_out, gF, c, b, a.flags[c], qv, dsdq[threadIdx.x]);
     return;
   }
@@ -501,6 +584,9 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjoint_collide(AdjArgs<typename 
 // (x, j), the low-branch neighbour term of link (x + e_j, opp j)), so one batch
 // covers every target; longer lists would take more batches.
 constexpr int SCATTER_BATCH = 4;
+// 64-thread workgroups: the few thousand targets spread over more CUs (96 x 48 x 48:
+// 2000-step dCd/dr 73.5 -> 72.8 ms against 256-thread workgroups).
+constexpr int kScatterBlock = 64;
 
 template <typename R, class LM>
 __global__ void k_adjoint_scatter(int n_targets, const int32_t* __restrict__ tgt, const int32_t* __restrict__ start,
@@ -550,6 +636,27 @@ __global__ void k_adjoint_scatter(int n_targets, const int32_t* __restrict__ tgt
   LM::st(lam, t, old + acc, lam_scale);
 }
 
+// Folded link additions of one launch into its output: one thread per link
+// cell adds the additions that launch stored (add, 19 x n_links) to the
+// cell's own entries.  Each link cell's entries are its own, so no two
+// threads touch the same entry.
+template <typename R, class LM>
+__global__ void k_link_finish(AdjArgs<R> a, const R* __restrict__ add, typename LM::T* __restrict__ lam) {
+  const int b = int(blockIdx.x * blockDim.x + threadIdx.x);
+  if (b >= a.n_links) return;
+  int x, y, z;
+  adj_coords(a, uint32_t(a.link_cell[b]), x, y, z);
+  const uint32_t o = adj_addr(a, x, y, z);
+  const size_t N = a.P;
+  sfor<Q>([&](auto I) {
+    constexpr int i = I;
+    if constexpr (i != 0) {
+      const size_t e = size_t(i) * N + o;
+      LM::st(lam, e, LM::ld(lam, e, a.lam_inv) + add[size_t(i) * a.n_links + b], a.lam_scale);
+    }
+  });
+}
+
 template <typename R>
 int make_adj_args(const lbm_step_desc* d, const lbm_adjoint_desc* ad, int mode, AdjArgs<R>& a) {
   if (!d || !ad || d->nx < 2 || d->ny <= 0 || d->nz <= 0) return LBM_E_ARG;
@@ -586,24 +693,32 @@ int make_adj_args(const lbm_step_desc* d, const lbm_adjoint_desc* ad, int mode, 
   a.mu_link = static_cast<R*>(ad->mu_link);
   a.phi_acc = ad->phi_acc;
   a.u_acc = ad->u_acc;
+  a.fold = ad->fold_links ? 1 : 0;
+  a.add_prev = nullptr;
   const double scale = (ad->lam_storage == LBM_F16DEV && ad->lam_scale > 0.0) ? ad->lam_scale : 1.0;
   a.lam_scale = R(scale);
   a.lam_inv = R(1.0 / scale);
   if (a.bounce == LBM_BOUNCE_HARD && !a.flags) return LBM_E_ARG;
   if (a.bounce == LBM_BOUNCE_SOFT && !a.phi) return LBM_E_ARG;
   if (a.n_links < 0 || (a.n_links > 0 && (!a.link_cell || !a.link_q || !a.flags || !a.mu_link))) return LBM_E_ARG;
-  a.nlb = (mode == LBM_MODE_COLLIDE) ? 0 : (a.n_links + kAdjBlock - 1) / kAdjBlock;
+  a.nlb = (mode == LBM_MODE_COLLIDE) ? 0 : (a.n_links + kLinkCellsPerBlock - 1) / kLinkCellsPerBlock;
   a.ndb = int((n + kAdjBlock - 1) / kAdjBlock);
   return LBM_OK;
 }
 
+// mu_cur (null: ad->mu_link) receives this launch's link-cell cotangents,
+// or with fold_links their additions to the cells' own entries; add_prev is
+// the previous launch's additions in a folded chain (else null).
 template <class S, class LM>
 int launch_adjoint(const lbm_step_desc* d, const lbm_adjoint_desc* ad, int mode, const void* state,
-                   const void* lam_in, void* lam_out, const double* gF, hipStream_t st) {
+                   const void* lam_in, void* lam_out, const double* gF, hipStream_t st,
+                   void* mu_cur = nullptr, const void* add_prev = nullptr) {
   using R = typename S::R;
   AdjArgs<R> a;
   const int rc = make_adj_args(d, ad, mode, a);
   if (rc) return rc;
+  if (mu_cur) a.mu_link = static_cast<R*>(mu_cur);
+  a.add_prev = static_cast<const R*>(add_prev);
   if (!state || !lam_in || !lam_out || lam_in == lam_out) return LBM_E_ARG;
   // the row-uniform sweep addresses both buffers through 32-bit byte offsets
   const long long extent = 1LL * (Q - 1) * a.P + (1LL * a.ny * a.nz - 1) * a.RS + a.nx;
@@ -615,7 +730,7 @@ int launch_adjoint(const lbm_step_desc* d, const lbm_adjoint_desc* ad, int mode,
   constexpr bool f32 = std::is_same<S, StoreF32>::value;
   const bool large = f32 && mode != LBM_MODE_COLLIDE && 1LL * a.N >= (1LL << 21);
   if (large) {
-    a.nlb = (a.n_links + kAdjBlockLarge - 1) / kAdjBlockLarge;
+    a.nlb = (a.n_links + kLinkCellsPerBlock - 1) / kLinkCellsPerBlock;
     a.ndb = int((1LL * a.N + kAdjBlockLarge - 1) / kAdjBlockLarge);
   }
   a.state_bytes = row ? uint32_t(sb) : 0u;
@@ -654,6 +769,56 @@ int launch_adjoint(const lbm_step_desc* d, const lbm_adjoint_desc* ad, int mode,
   return hipGetLastError() == hipSuccess ? LBM_OK : LBM_E_LAUNCH;
 }
 
+template <class S, class LM>
+int launch_finish(const lbm_step_desc* d, const lbm_adjoint_desc* ad, const void* mu, void* lam, hipStream_t st) {
+  using R = typename S::R;
+  AdjArgs<R> a;
+  const int rc = make_adj_args(d, ad, LBM_MODE_FULL, a);
+  if (rc) return rc;
+  if (a.n_links == 0) return LBM_OK;
+  if (!mu || !lam) return LBM_E_ARG;
+  const unsigned grid = unsigned((a.n_links + kScatterBlock - 1) / kScatterBlock);
+  hipLaunchKernelGGL((k_link_finish<R, LM>), grid, kScatterBlock, 0, st, a, static_cast<const R*>(mu),
+                     static_cast<typename LM::T*>(lam));
+  return hipGetLastError() == hipSuccess ? LBM_OK : LBM_E_LAUNCH;
+}
+
+// Dispatch on the population storage and the cotangent storage.
+template <template <class, class> class F, typename... A>
+int adj_dispatch(const lbm_step_desc* d, const lbm_adjoint_desc* ad, A... args) {
+  if (ad->lam_storage == LBM_F16DEV) {
+    switch (d->storage) {
+      case LBM_F32: return F<StoreF32, LamHalf>::run(d, ad, args...);
+      case LBM_F16DEV: return F<StoreF16, LamHalf>::run(d, ad, args...);
+      case LBM_BF16DEV: return F<StoreBF16, LamHalf>::run(d, ad, args...);
+    }
+    return LBM_E_DTYPE;
+  }
+  if (ad->lam_storage != 0) return LBM_E_DTYPE;
+  switch (d->storage) {
+    case LBM_F32: return F<StoreF32, LamPlain<float>>::run(d, ad, args...);
+    case LBM_F64: return F<StoreF64, LamPlain<double>>::run(d, ad, args...);
+    case LBM_F16DEV: return F<StoreF16, LamPlain<float>>::run(d, ad, args...);
+    case LBM_BF16DEV: return F<StoreBF16, LamPlain<float>>::run(d, ad, args...);
+  }
+  return LBM_E_DTYPE;
+}
+
+template <class S, class LM>
+struct AdjointLaunch {
+  static int run(const lbm_step_desc* d, const lbm_adjoint_desc* ad, int mode, const void* state, const void* lam_in,
+                 void* lam_out, const double* gF, hipStream_t st, void* mu_cur, const void* add_prev) {
+    return launch_adjoint<S, LM>(d, ad, mode, state, lam_in, lam_out, gF, st, mu_cur, add_prev);
+  }
+};
+
+template <class S, class LM>
+struct FinishLaunch {
+  static int run(const lbm_step_desc* d, const lbm_adjoint_desc* ad, const void* mu, void* lam, hipStream_t st) {
+    return launch_finish<S, LM>(d, ad, mu, lam, st);
+  }
+};
+
 }  // namespace lbm
 
 using namespace lbm;
@@ -662,25 +827,16 @@ extern "C" {
 
 int lbm_step_adjoint(const lbm_step_desc* d, const lbm_adjoint_desc* ad, int mode, const void* state,
                      const void* lam_in, void* lam_out, const double* gF, lbm_stream_t stream) {
-  if (!d) return LBM_E_ARG;
-  hipStream_t s = (hipStream_t)stream;
-  if (!ad) return LBM_E_ARG;
-  if (ad->lam_storage == LBM_F16DEV) {
-    switch (d->storage) {
-      case LBM_F32: return launch_adjoint<StoreF32, LamHalf>(d, ad, mode, state, lam_in, lam_out, gF, s);
-      case LBM_F16DEV: return launch_adjoint<StoreF16, LamHalf>(d, ad, mode, state, lam_in, lam_out, gF, s);
-      case LBM_BF16DEV: return launch_adjoint<StoreBF16, LamHalf>(d, ad, mode, state, lam_in, lam_out, gF, s);
-    }
-    return LBM_E_DTYPE;
-  }
-  if (ad->lam_storage != 0) return LBM_E_DTYPE;
-  switch (d->storage) {
-    case LBM_F32: return launch_adjoint<StoreF32, LamPlain<float>>(d, ad, mode, state, lam_in, lam_out, gF, s);
-    case LBM_F64: return launch_adjoint<StoreF64, LamPlain<double>>(d, ad, mode, state, lam_in, lam_out, gF, s);
-    case LBM_F16DEV: return launch_adjoint<StoreF16, LamPlain<float>>(d, ad, mode, state, lam_in, lam_out, gF, s);
-    case LBM_BF16DEV: return launch_adjoint<StoreBF16, LamPlain<float>>(d, ad, mode, state, lam_in, lam_out, gF, s);
-  }
-  return LBM_E_DTYPE;
+  if (!d || !ad) return LBM_E_ARG;
+  return adj_dispatch<AdjointLaunch>(d, ad, mode, state, lam_in, lam_out, gF, (hipStream_t)stream,
+                                     static_cast<void*>(nullptr), static_cast<const void*>(nullptr));
+}
+
+int lbm_adjoint_link_finish(const lbm_step_desc* d, const lbm_adjoint_desc* ad, const void* mu, void* lam,
+                            lbm_stream_t stream) {
+  if (!d || !ad || !ad->fold_links) return LBM_E_ARG;
+  return adj_dispatch<FinishLaunch>(d, ad, mu ? mu : static_cast<const void*>(ad->mu_link), lam,
+                                    (hipStream_t)stream);
 }
 
 int lbm_adjoint_scatter(const lbm_step_desc* d, const lbm_adjoint_desc* ad, void* lam, const double* gF,
@@ -688,22 +844,22 @@ int lbm_adjoint_scatter(const lbm_step_desc* d, const lbm_adjoint_desc* ad, void
   if (!d || !ad || !lam) return LBM_E_ARG;
   if (ad->n_targets <= 0) return LBM_OK;
   if (!ad->tgt_index || !ad->tgt_start || !ad->src || !ad->coef || !ad->mu_link) return LBM_E_ARG;
-  const unsigned grid = unsigned((ad->n_targets + 255) / 256);
+  const unsigned grid = unsigned((ad->n_targets + kScatterBlock - 1) / kScatterBlock);
   hipStream_t s = (hipStream_t)stream;
   if (ad->lam_storage == LBM_F16DEV) {
     if (d->storage == LBM_F64) return LBM_E_DTYPE;
     const float scale = ad->lam_scale > 0.0 ? float(ad->lam_scale) : 1.0f;
-    hipLaunchKernelGGL((k_adjoint_scatter<float, LamHalf>), grid, 256, 0, s, ad->n_targets, ad->tgt_index,
+    hipLaunchKernelGGL((k_adjoint_scatter<float, LamHalf>), grid, kScatterBlock, 0, s, ad->n_targets, ad->tgt_index,
                        ad->tgt_start, ad->src, ad->coef, static_cast<const float*>(ad->mu_link), gF,
                        static_cast<unsigned short*>(lam), scale, 1.0f / scale);
   } else if (ad->lam_storage != 0) {
     return LBM_E_DTYPE;
   } else if (d->storage == LBM_F64) {
-    hipLaunchKernelGGL((k_adjoint_scatter<double, LamPlain<double>>), grid, 256, 0, s, ad->n_targets, ad->tgt_index,
+    hipLaunchKernelGGL((k_adjoint_scatter<double, LamPlain<double>>), grid, kScatterBlock, 0, s, ad->n_targets, ad->tgt_index,
                        ad->tgt_start, ad->src, ad->coef, static_cast<const double*>(ad->mu_link), gF,
                        static_cast<double*>(lam), 1.0, 1.0);
   } else {
-    hipLaunchKernelGGL((k_adjoint_scatter<float, LamPlain<float>>), grid, 256, 0, s, ad->n_targets, ad->tgt_index,
+    hipLaunchKernelGGL((k_adjoint_scatter<float, LamPlain<float>>), grid, kScatterBlock, 0, s, ad->n_targets, ad->tgt_index,
                        ad->tgt_start, ad->src, ad->coef, static_cast<const float*>(ad->mu_link), gF,
                        static_cast<float*>(lam), 1.0f, 1.0f);
   }
@@ -714,14 +870,34 @@ int lbm_adjoint_scatter(const lbm_step_desc* d, const lbm_adjoint_desc* ad, void
 int lbm_adjoint_chain(const lbm_step_desc* d, const lbm_adjoint_desc* ad, int n, const void* const* states,
                       void* lam_a, void* lam_b, const double* gF, lbm_stream_t stream) {
   if (!d || !ad || n < 0 || (n > 0 && (!states || !lam_a || !lam_b)) || lam_a == lam_b) return LBM_E_ARG;
+  if (ad->fold_links && d->n_links > 0 && (!ad->mu_link || !ad->mu_link_alt || ad->mu_link == ad->mu_link_alt))
+    return LBM_E_ARG;
+  hipStream_t st = (hipStream_t)stream;
   void* cur = lam_a;
   void* nxt = lam_b;
+  // folded: each launch adds the previous launch's link additions to what it
+  // loads and stores its own in the other mu buffer; the chain's last launch
+  // is completed by lbm_adjoint_link_finish
+  void* mu[2] = {ad->mu_link, ad->mu_link_alt};
   for (int k = n - 1; k >= 0; --k) {
     const double* g = gF ? gF + 3 * size_t(k) : nullptr;
-    int rc = lbm_step_adjoint(d, ad, LBM_MODE_FULL, states[k], cur, nxt, g, stream);
-    if (rc) return rc;
-    rc = lbm_adjoint_scatter(d, ad, nxt, g, stream);
-    if (rc) return rc;
+    int rc;
+    if (ad->fold_links) {
+      const int w = (n - 1 - k) & 1;
+      const bool first = (k == n - 1);
+      rc = adj_dispatch<AdjointLaunch>(d, ad, int(LBM_MODE_FULL), states[k], static_cast<const void*>(cur), nxt, g, st,
+                                       mu[w], first ? static_cast<const void*>(nullptr) : mu[w ^ 1]);
+      if (rc) return rc;
+      if (k == 0) {
+        rc = lbm_adjoint_link_finish(d, ad, mu[w], nxt, stream);
+        if (rc) return rc;
+      }
+    } else {
+      rc = lbm_step_adjoint(d, ad, LBM_MODE_FULL, states[k], cur, nxt, g, stream);
+      if (rc) return rc;
+      rc = lbm_adjoint_scatter(d, ad, nxt, g, stream);
+      if (rc) return rc;
+    }
     void* t = cur;
     cur = nxt;
     nxt = t;

@@ -71,20 +71,24 @@ __device__ __forceinline__ void feq_vjp(R h0, const R* hs, const R* hd, const Ma
 
 // BGK: c = s - omega (s - feq(s)).  g <- J^T g; returns dL/domega.
 // J^T g = (1 - omega) g + omega (d feq / d s)^T g.
-template <typename R>
+template <bool GOM, typename R>
 __device__ __forceinline__ R bgk_vjp(const Pairs<R>& ps, const Macro<R>& m, R* g, R omega) {
   const Pairs<R> pg = to_pairs(g);
   // dL/domega = -sum_i g_i (s_i - feq_i), per pair
   //   = -(gs (ss - 2 sym) + gd (sd - 2 asym)) / 2
-  R gom = -pg.f0 * (ps.f0 - R(weight(0)) * m.rho * (R(1) - R(1.5) * m.usq));
-  R acc = R(0);
-  sfor<NP>([&](auto P) {
-    constexpr int k = P;
-    R sym, asym;
-    feq_pair<k>(m, sym, asym);
-    acc += pg.s[k] * (ps.s[k] - R(2) * sym) + pg.d[k] * (ps.d[k] - R(2) * asym);
-  });
-  gom -= R(0.5) * acc;
+  // (only computed when a caller uses it: GOM)
+  R gom = R(0);
+  if constexpr (GOM) {
+    gom = -pg.f0 * (ps.f0 - R(weight(0)) * m.rho * (R(1) - R(1.5) * m.usq));
+    R acc = R(0);
+    sfor<NP>([&](auto P) {
+      constexpr int k = P;
+      R sym, asym;
+      feq_pair<k>(m, sym, asym);
+      acc += pg.s[k] * (ps.s[k] - R(2) * sym) + pg.d[k] * (ps.d[k] - R(2) * asym);
+    });
+    gom -= R(0.5) * acc;
+  }
   R Gr, Gx, Gy, Gz;
   feq_vjp<true>(pg.f0, pg.s, pg.d, m, Gr, Gx, Gy, Gz);
   const R k1 = R(1) - omega;
@@ -117,7 +121,7 @@ __device__ __forceinline__ void meq_vjp(const R* z, const Macro<R>& m, R& Grho, 
 
 // MRT: c = s - M^T D (M s - meq(s)), D = diag(rate_k / ||M_k||^2).
 // J^T g = g - M^T h + (d meq/d s)^T h with h = D M g.  Returns dL/domega.
-template <typename R>
+template <bool GOM, typename R>
 __device__ __forceinline__ R mrt_vjp(const Pairs<R>& ps, const Macro<R>& mac, R* g, R omega) {
   const Pairs<R> pg = to_pairs(g);
   R h[Q];
@@ -130,8 +134,10 @@ __device__ __forceinline__ R mrt_vjp(const Pairs<R>& ps, const Macro<R>& mac, R*
       constexpr double inv_n = 1.0 / double(mrt_norm(k));
       const R Mg = moment<k>(pg);
       if constexpr (mrt_is_stress(k)) {
-        const R dm = moment<k>(ps) - meq<k>(mac.rho, mac.ux, mac.uy, mac.uz, mac.usq);
-        gom -= Mg * dm * R(inv_n);
+        if constexpr (GOM) {
+          const R dm = moment<k>(ps) - meq<k>(mac.rho, mac.ux, mac.uy, mac.uz, mac.usq);
+          gom -= Mg * dm * R(inv_n);
+        }
         h[k] = omega * Mg * R(inv_n);
       } else {
         h[k] = R(mrt_rate_fixed(k)) * Mg * R(inv_n);

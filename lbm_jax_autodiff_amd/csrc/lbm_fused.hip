@@ -8,17 +8,17 @@
 // writing P_{t+1} once: 2 x 19 values per cell.
 //
 // One launch per step.  Its first workgroups own the sparse link cells
-// (fluid cells with a Bouzidi link, listed in link_cell/link_q) and produce
-// the float64 momentum-exchange partials; the rest sweep the dense grid in
-// workgroup order and skip link cells.  Hardware dispatch deals consecutive
-// workgroups to the 8 XCDs in turn, so at any moment all XCDs update
-// neighbouring rows and the row-interleaved state is read and written in a
-// narrow address window (remapping tiles so that each XCD sweeps its own
-// contiguous eighth of the grid was 6% slower).  The link path is written to fit the dense sweep's register budget
-// (link_cell_update): a link path that needs more either lowers the occupancy
-// of every workgroup or, held to the budget, spills and finishes last under
-// the full memory load; as a separate launch ahead of the sweep it cost
-// 15 us plus a launch gap per step at 256 x 128 x 128.  With 16-bit
+// (fluid cells with a Bouzidi link, listed in link_cell/link_q), 64 per
+// workgroup, and produce the float64 momentum-exchange partials; the rest
+// sweep the dense grid in workgroup order and skip link cells.  Hardware
+// dispatch deals consecutive workgroups to the 8 XCDs in turn, so at any
+// moment all XCDs update neighbouring rows and the row-interleaved state is
+// read and written in a narrow address window (remapping tiles so that each
+// XCD sweeps its own contiguous eighth of the grid was 6% slower).  A link
+// workgroup spreads its cells' Bouzidi pulls over its waves by direction and
+// then collides on one wave (link_block), so on small grids, where the step
+// waits for the link cells, a link cell costs two dependent round trips and
+// a fraction of the address work before its collision.  With 16-bit
 // populations the sweep is bound by vector instructions rather than memory,
 // and where nx is a multiple of 64 it takes each wave's row addresses from
 // scalar registers (dense_row).
@@ -346,102 +346,158 @@ constexpr int step_waves_per_eu() {
   return V > 1 ? 1 : (sizeof(typename S::R) == 8 ? 3 : 6);
 }
 
-// One link cell b: Bouzidi pulls (streaming.py:113-181), Zou-He, the cell's
-// momentum-exchange contribution (forces.py:95-118) written to f_out,
-// collision, store.  Written for the dense sweep's register budget: the cell
-// index and the 18 q values are loaded together, then the 18 directions are
-// processed in chunks of CH, each chunk issuing its loads (per direction the
-// cell's own P_opp(i), and the Bouzidi second neighbour P(x + e_i)_opp, the
-// cell's own P_i or the normal pull) at once.  Every load is unconditional
-// (all addresses are in range), so a link cell costs 1 + 18/CH dependent
-// round trips.  The own part of the momentum exchange is accumulated as the
-// values arrive, the pulled part after Zou-He, and the partial sums leave
-// the registers before the collision.
-template <class S, int COLL, int MODE, int CH>
-__device__ __forceinline__ void link_cell_update(const StepArgs<typename S::R>& a,
-                                               const typename S::T* __restrict__ src,
-                                               typename S::T* __restrict__ dst, int b, double* f_out) {
+// Phase 1 of a link workgroup, on the wave that owns the directions i with
+// i % NW == W: for its link cell (streaming.py:113-181) it loads q_opp(i),
+// then at once the cell's own P_opp(i) and the second value (the Bouzidi
+// neighbour P(x + e_i)_opp below q = 1/2, the cell's own P_i above, the
+// normal pull where opp(i) is no wall link), forms S_i into LDS, adds the own
+// part e_opp(i) P_opp(i) of the momentum exchange (forces.py:95-118) to f and
+// sets bit opp(i) of mask where opp(i) is a wall link.  Straight-line code
+// over the wave's few directions, so its loads go out in two rounds (q, then
+// the populations).
+template <class S, int W, int NW>
+__device__ __forceinline__ void link_pull_part(const StepArgs<typename S::R>& a, const typename S::T* __restrict__ src,
+                                               int b, int x, int y, int z, uint32_t o, const Rows& rows,
+                                               typename S::R (*sl)[kLinkCellsPerBlock], int lane, double* f,
+                                               uint32_t& mask) {
   using R = typename S::R;
+  constexpr int ND = (Q - W + NW - 1) / NW;  // directions W, W + NW, ...
   const size_t N = a.P;
-  double f[3] = {0.0, 0.0, 0.0};
-  R qv[Q];
-  const uint32_t c = uint32_t(a.link_cell[b]);
-  sfor<Q>([&](auto I) {
-    constexpr int i = I;
-    if constexpr (i != 0) qv[i] = R(a.link_q[size_t(i - 1) * a.n_links + b]);
+  R qo[ND], va[ND], vb[ND];
+  sfor<ND>([&](auto K) {
+    constexpr int i = W + int(K) * NW;
+    if constexpr (i != 0) qo[K] = R(a.link_q[size_t(opp(i) - 1) * a.n_links + b]);
   });
-  // pin the converted q in registers here: left to its uses, the
-  // conversion sinks and the 18 float64 values (36 registers) stay alive
-  // through the pulls
-#pragma unroll
-  for (int i = 1; i < Q; ++i) asm volatile("" : "+v"(qv[i]));
-  int x, y, z;
-  cell_coords(a, c, x, y, z);
-  const uint32_t o = cell_addr(a, x, y, z);
-  const Rows rows = make_rows(a, y, z);
-  R s[Q];
-  s[0] = S::ld_c(src, o, a.off[0]);
-  sfor<(Q - 1 + CH - 1) / CH>([&](auto K) {
-    constexpr int k0 = 1 + int(K) * CH;
-    R va[CH], vb[CH];
-    sfor<CH>([&](auto J) {
-      constexpr int i = k0 + int(J);
-      if constexpr (i < Q) {
-        constexpr int io = opp(i);
-        const R qo = qv[io];
-        const bool low = (qo >= R(0)) & (qo < R(0.5));
-        const bool high = qo >= R(0.5);
-        const size_t ka = low ? io * N + wrap_next<i>(a, x, y, z)
-                              : (high ? i * N + o : i * N + pull_src<i>(a, rows, o, x));
-        va[J] = S::ld_c(src, ka, low ? a.off[io] : a.off[i]);
-        vb[J] = S::ld_c(src, io * N + o, a.off[io]);
-      }
-    });
-    sfor<CH>([&](auto J) {
-      constexpr int i = k0 + int(J);
-      if constexpr (i < Q) {
-        constexpr int io = opp(i);
-        const R qo = qv[io];
-        if (qo >= R(0.5)) {
-          const R inv2q = R(1) / (R(2) * fmin(qo, R(1)));
-          s[i] = inv2q * vb[J] + (R(1) - inv2q) * va[J];
-        } else if (qo >= R(0)) {
-          const R twoq = R(2) * qo;
-          s[i] = twoq * vb[J] + (R(1) - twoq) * va[J];
-        } else {
-          s[i] = va[J];
-        }
-        // own part of the momentum exchange for link j = io
-        if (qo >= R(0)) {
-          const double t = double(vb[J]);
-          add_scaled<EX[io]>(f[0], t);
-          add_scaled<EY[io]>(f[1], t);
-          add_scaled<EZ[io]>(f[2], t);
-        }
-      }
-    });
-    // keep the next chunk's loads from being hoisted above this one
-    __builtin_amdgcn_sched_barrier(0);
+  sfor<ND>([&](auto K) {
+    constexpr int i = W + int(K) * NW;
+    if constexpr (i == 0) {
+      va[K] = S::ld_c(src, o, a.off[0]);
+    } else {
+      constexpr int io = opp(i);
+      const bool low = (qo[K] >= R(0)) & (qo[K] < R(0.5));
+      const bool high = qo[K] >= R(0.5);
+      const size_t ka = low ? io * N + wrap_next<i>(a, x, y, z)
+                            : (high ? i * N + o : i * N + pull_src<i>(a, rows, o, x));
+      va[K] = S::ld_c(src, ka, low ? a.off[io] : a.off[i]);
+      vb[K] = S::ld_c(src, io * N + o, a.off[io]);
+    }
   });
-  boundaries(a, s, x);
-  sfor<Q>([&](auto J) {
-    constexpr int j = J;
-    if constexpr (j != 0) {
-      if (qv[j] >= R(0)) {
-        const double t = double(s[opp(j)]);
-        add_scaled<EX[j]>(f[0], t);
-        add_scaled<EY[j]>(f[1], t);
-        add_scaled<EZ[j]>(f[2], t);
+  sfor<ND>([&](auto K) {
+    constexpr int i = W + int(K) * NW;
+    if constexpr (i == 0) {
+      sl[0][lane] = va[K];
+    } else {
+      constexpr int io = opp(i);
+      const R q = qo[K];
+      R si;
+      if (q >= R(0.5)) {
+        const R inv2q = R(1) / (R(2) * fmin(q, R(1)));
+        si = inv2q * vb[K] + (R(1) - inv2q) * va[K];
+      } else if (q >= R(0)) {
+        const R twoq = R(2) * q;
+        si = twoq * vb[K] + (R(1) - twoq) * va[K];
+      } else {
+        si = va[K];
+      }
+      sl[i][lane] = si;
+      if (q >= R(0)) {
+        mask |= 1u << io;
+        const double t = double(vb[K]);
+        add_scaled<EX[io]>(f[0], t);
+        add_scaled<EY[io]>(f[1], t);
+        add_scaled<EZ[io]>(f[2], t);
       }
     }
   });
-  // hand the partial sums over before the collision, which needs every
-  // register the dense budget leaves
-  f_out[0] = f[0];
-  f_out[1] = f[1];
-  f_out[2] = f[2];
-  if (MODE == LBM_MODE_FULL) collide_cell<R, COLL>(a, s, c, a.flags[c]);
-  store_cell<S>(a, dst, o, s);
+}
+
+// A link workgroup: kLinkCellsPerBlock link cells, lane l of every wave on
+// link b = first + l.  Phase 1 (all waves): the Bouzidi pulls, spread over
+// the waves by direction (link_pull_part).  Phase 2 (wave 0): per cell,
+// Zou-He, the pulled part e_j S_opp(j) of the momentum exchange, collision
+// and store.  The workgroup's force partial (float64) goes to its slot of
+// fpart.  A link cell so waits two round trips before its collision, and no
+// wave runs all 18 directions' address and branch work.
+template <class S, int COLL, int MODE, int BS>
+__device__ __forceinline__ void link_block(const StepArgs<typename S::R>& a, const typename S::T* __restrict__ src,
+                                           typename S::T* __restrict__ dst, uint32_t bid,
+                                           double* __restrict__ fpart) {
+  using R = typename S::R;
+  constexpr int NW = BS / 64;
+  static_assert(kLinkCellsPerBlock == 64, "one link cell per lane");
+  __shared__ R sl[Q][kLinkCellsPerBlock];
+  __shared__ uint32_t lm[NW][kLinkCellsPerBlock];
+  __shared__ double red[NW][3];
+  const int lane = int(threadIdx.x & 63u);
+  const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+  const int b = int(bid) * kLinkCellsPerBlock + lane;
+  const bool live = b < a.n_links;
+  double f[3] = {0.0, 0.0, 0.0};
+  uint32_t mask = 0, c = 0, fl = 0, o = 0;
+  int x = 0, y = 0, z = 0;
+  if (live) {
+    c = uint32_t(a.link_cell[b]);
+    cell_coords(a, c, x, y, z);
+    o = cell_addr(a, x, y, z);
+    if (w == 0 && MODE == LBM_MODE_FULL) fl = a.flags[c];
+    const Rows rows = make_rows(a, y, z);
+    sfor<NW>([&](auto W) {
+      if (w == int(W)) link_pull_part<S, int(W), NW>(a, src, b, x, y, z, o, rows, sl, lane, f, mask);
+    });
+  }
+  lm[w][lane] = mask;
+  if (fpart) {
+    const double fx = wave_sum(f[0]), fy = wave_sum(f[1]), fz = wave_sum(f[2]);
+    if (lane == 0) {
+      red[w][0] = fx;
+      red[w][1] = fy;
+      red[w][2] = fz;
+    }
+  }
+  __syncthreads();
+  if (w != 0) return;
+  R s[Q];
+  double g[3] = {0.0, 0.0, 0.0};
+  if (live) {
+#pragma unroll
+    for (int k = 1; k < NW; ++k) mask |= lm[k][lane];
+    sfor<Q>([&](auto I) {
+      constexpr int i = I;
+      s[i] = sl[i][lane];
+    });
+    boundaries(a, s, x);
+    sfor<Q>([&](auto J) {
+      constexpr int j = J;
+      if constexpr (j != 0) {
+        if (mask & (1u << j)) {
+          const double t = double(s[opp(j)]);
+          add_scaled<EX[j]>(g[0], t);
+          add_scaled<EY[j]>(g[1], t);
+          add_scaled<EZ[j]>(g[2], t);
+        }
+      }
+    });
+  }
+  // the pulled part joins wave 0's partial before the collision, which needs
+  // every register the dense budget leaves
+  if (fpart) {
+    const double gx = wave_sum(g[0]), gy = wave_sum(g[1]), gz = wave_sum(g[2]);
+    if (lane == 0) {
+      red[0][0] += gx;
+      red[0][1] += gy;
+      red[0][2] += gz;
+    }
+  }
+  if (live) {
+    if (MODE == LBM_MODE_FULL) collide_cell<R, COLL>(a, s, c, fl);
+    store_cell<S>(a, dst, o, s);
+  }
+  if (fpart && lane < 3) {
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) acc += red[k][lane];
+    fpart[size_t(bid) * 3 + lane] = acc;
+  }
 }
 
 template <class S, int COLL, int MODE, int V, int BS = kStepBlock>
@@ -453,30 +509,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(step_waves_p
   const uint32_t bid = blockIdx.x;
 
   if (MODE != LBM_MODE_COLLIDE && int(bid) < a.nlb) {
-    // ---------------- link cells ----------------
-    __shared__ double fl[BS][3];
-    const int b = int(bid) * BS + int(threadIdx.x);
-    fl[threadIdx.x][0] = fl[threadIdx.x][1] = fl[threadIdx.x][2] = 0.0;
-    if (b < a.n_links) link_cell_update<S, COLL, MODE, 6>(a, src, dst, b, fl[threadIdx.x]);
-    if (fpart) {
-      __shared__ double red[3][BS / 64];
-      const double fx = wave_sum(fl[threadIdx.x][0]);
-      const double fy = wave_sum(fl[threadIdx.x][1]);
-      const double fz = wave_sum(fl[threadIdx.x][2]);
-      const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-      if (lane == 0) {
-        red[0][w] = fx;
-        red[1][w] = fy;
-        red[2][w] = fz;
-      }
-      __syncthreads();
-      if (threadIdx.x < 3) {
-        double acc = 0.0;
-#pragma unroll
-        for (int k = 0; k < BS / 64; ++k) acc += red[threadIdx.x][k];
-        fpart[size_t(bid) * 3 + threadIdx.x] = acc;
-      }
-    }
+    link_block<S, COLL, MODE, BS>(a, src, dst, bid, fpart);
     return;
   }
 
@@ -589,7 +622,7 @@ int make_args(const lbm_step_desc* d, int mode, StepArgs<R>& a) {
   if (a.n_links < 0 || (a.n_links > 0 && (!a.link_cell || !a.link_q || !a.flags))) return LBM_E_ARG;
   if (a.coll != LBM_COLL_BGK && a.coll != LBM_COLL_MRT) return LBM_E_ARG;
   if (a.guard < 0 || a.guard > 2 || a.bounce < 0 || a.bounce > 2) return LBM_E_ARG;
-  a.nlb = (mode == LBM_MODE_COLLIDE) ? 0 : (a.n_links + kStepBlock - 1) / kStepBlock;
+  a.nlb = (mode == LBM_MODE_COLLIDE) ? 0 : (a.n_links + kLinkCellsPerBlock - 1) / kLinkCellsPerBlock;
   a.ndb = int((n + kStepBlock - 1) / kStepBlock);
   a.row_waves = 0;
   a.state_bytes = 0;
@@ -610,7 +643,7 @@ inline int choose_vec(const lbm_step_desc* d, uint32_t pstride, uint32_t rstride
 // float32 sweep of grids of 2^21 cells or more (1% faster than 256 at
 // 256 x 128 x 128; 768 and 1024 are slower), 256 otherwise (the 16-bit sweep
 // is 7% slower at 512, and small grids lose parallelism: 96 x 48 x 48 went
-// from 14.4 to 18.4 us per step).  Also sets the number of force slots.
+// from 14.4 to 18.4 us per step).
 inline int step_block(const lbm_step_desc* d) {
   long long ps = 0, rs = 0;
   if (!state_layout(d, ps, rs)) return kStepBlock;
@@ -655,7 +688,7 @@ int launch_step(const lbm_step_desc* d, int mode, const void* src, void* dst, do
     a.state_bytes = a.row_waves ? uint32_t(bytes) : 0u;
   }
   const int bs = step_block(d);
-  a.nlb = (mode == LBM_MODE_COLLIDE) ? 0 : (a.n_links + bs - 1) / bs;
+  a.nlb = (mode == LBM_MODE_COLLIDE) ? 0 : (a.n_links + kLinkCellsPerBlock - 1) / kLinkCellsPerBlock;
   const long long cells_per_block = 1LL * bs * V;
   a.ndb = int((1LL * a.N + cells_per_block - 1) / cells_per_block);
   const auto* in = static_cast<const typename S::T*>(src);
@@ -724,8 +757,7 @@ int64_t lbm_state_stride(const lbm_step_desc* d) {
 
 int lbm_step_force_slots(const lbm_step_desc* d) {
   if (!d || d->n_links < 0) return 0;
-  const int bs = step_block(d);
-  return (d->n_links + bs - 1) / bs;
+  return (d->n_links + kLinkCellsPerBlock - 1) / kLinkCellsPerBlock;
 }
 
 int lbm_step(const lbm_step_desc* d, int mode, const void* src, void* dst, double* force_partial,

@@ -111,7 +111,7 @@ __global__ void k_collide_vjp(const R* __restrict__ fin, R tau, const R* __restr
   const Pairs<R> ps = to_pairs(f);
   const Macro<R> m = macro(ps);
   const R omega = R(1) / (tau_field ? tau_field[c] : tau);
-  const R gom = MRT ? mrt_vjp(ps, m, g, omega) : bgk_vjp(ps, m, g, omega);
+  const R gom = MRT ? mrt_vjp<true>(ps, m, g, omega) : bgk_vjp<true>(ps, m, g, omega);
   vstore(gf, N, c, g);
   if (gtau) gtau[c] = -gom * omega * omega;
 }

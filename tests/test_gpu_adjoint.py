@@ -235,6 +235,40 @@ def test_checkpointed_gradient_equals_store_all(dev, policy):
     assert torch.equal(out[0], out[1])
 
 
+@pytest.mark.parametrize("storage,tol", [("f64", 1e-13), ("f32", 2e-6), ("f16", 2e-3)])
+@pytest.mark.parametrize("checkpoint", ["all", (7, 3)])
+def test_folded_link_additions_match_scatter(dev, storage, tol, checkpoint):
+    """Folding each adjoint step's Bouzidi and force additions into the next
+    step's loads gives the gradients of the separate per-entry scatter (the
+    same sums in another order), for f0, q, porosity and the inlet velocity;
+    high and low branches, soft and hard bounce, a chain of several steps."""
+    nx, ny, nz = 24, 12, 12
+    dt = torch.float64 if storage == "f64" else torch.float32
+    por, solid, q = lbm.smooth_sphere_geometry(nx, ny, nz, (-1.0, 0.1, 0.0), 0.6, 10.0, device=dev, out_dtype=dt)
+    u = torch.zeros((3, nx, ny, nz), dtype=dt, device=dev)
+    u[0] = 0.05
+    f0 = lbm.equilibrium(torch.ones((nx, ny, nz), dtype=dt, device=dev), u)
+    qv = q[q >= 0]
+    assert bool((qv < 0.5).any()) and bool((qv >= 0.5).any())
+    for bounce in ("soft", "hard"):
+        grads = []
+        for mode in ("scatter", "fold"):
+            ff = f0.clone().requires_grad_(True)
+            qq = q.clone().requires_grad_(True)
+            pp = por.clone().requires_grad_(True)
+            vel = torch.tensor([0.05, 0.0, 0.0], dtype=torch.float64, device=dev, requires_grad=True)
+            kw = dict(porosity=pp) if bounce == "soft" else dict(solid=solid)
+            S, P, F = A.rollout(ff, 17, tau=0.56, q=qq, inlet_velocity=vel, collision="mrt", guard="pre",
+                                guard_velocity=(0.05, 0.0, 0.0), storage=storage, offset="inlet",
+                                checkpoint=checkpoint, link_additions=mode, **kw)
+            loss = F[:, 0].sum() + (S.double() ** 2).sum() * 1e-3
+            wrt = [ff, qq, vel] + ([pp] if bounce == "soft" else [])
+            grads.append([cpu(g).double() for g in torch.autograd.grad(loss, wrt)])
+        for a, b in zip(*grads):
+            scale = max(1.0, float(a.abs().max()))
+            assert float((a - b).abs().max()) <= tol * scale, (bounce, float((a - b).abs().max()), scale)
+
+
 @pytest.mark.parametrize("storage", ["f32", "f16", "f64"])
 def test_gradient_independent_of_state_layout(dev, storage):
     """Row-interleaved and populations-apart buffers give bit-identical
