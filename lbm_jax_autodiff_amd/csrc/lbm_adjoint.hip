@@ -10,9 +10,12 @@
 // those populations, so they carry no cotangent) or where a Bouzidi link
 // replaces the pull.  Every entry of lambda_t is written exactly once, with
 // no atomics.  The extra consumers -- Bouzidi links (P(x)_iopp, P(x)_i,
-// P(x+e_i)_iopp) and the momentum-exchange force (P(x)_j) -- are added
-// afterwards by lbm_adjoint_scatter from a fixed per-entry plan, so the
-// result is deterministic.
+// P(x+e_i)_iopp) and the momentum-exchange force (P(x)_j) -- belong to the
+// link cells: the P(x+e_i)_iopp term joins the link cell's own owner write of
+// that entry, and the terms on the link cell's own populations are stored per
+// link and added by the next launch as it loads them (lbm_adjoint_link_finish
+// after a chain's last launch), or added by lbm_adjoint_scatter from a fixed
+// per-entry plan.  Either way the result is deterministic.
 #include "lbm_adjoint_physics.hpp"
 
 namespace lbm {
@@ -151,85 +154,28 @@ __device__ __forceinline__ void pull_dense(const AdjArgs<typename S::R>& a, cons
   });
 }
 
-// Pulled values of a link cell with Bouzidi (streaming.py:113-181), and
-// dS_i/dq of every link direction into dsdq (0 elsewhere and outside the
-// clip bounds).  qv holds q of the 18 directions.  The directions are
-// processed in chunks of CH, each issuing its loads together (the cell's own
-// P_opp(i), and the second neighbour P(x + e_i)_opp, the own P_i or the normal
-// pull); all loads are unconditional, so the cell costs 18/CH dependent
-// round trips and keeps no per-direction state in registers afterwards.
-template <class S, int CH>
-__device__ __forceinline__ void pull_link(const AdjArgs<typename S::R>& a, const typename S::T* __restrict__ P,
-                                          uint32_t o, int x, int y, int z, const typename S::R* qv,
-                                          typename S::R* s, typename S::R* dsdq) {
-  using R = typename S::R;
-  const size_t N = a.P;
-  s[0] = S::ld_c(P, o, a.off[0]);
-  sfor<(Q - 1 + CH - 1) / CH>([&](auto K) {
-    constexpr int k0 = 1 + int(K) * CH;
-    R va[CH], vb[CH];
-    sfor<CH>([&](auto J) {
-      constexpr int i = k0 + int(J);
-      if constexpr (i < Q) {
-        constexpr int io = opp(i);
-        const R qo = qv[io];
-        const bool low = (qo >= R(0)) & (qo < R(0.5));
-        const bool high = qo >= R(0.5);
-        const size_t normal = i * N + (x_fallback<i>(x, a.nx) ? o : owner_target<i>(a, x, y, z));
-        const size_t ka = low ? io * N + next_cell<i>(a, x, y, z) : (high ? i * N + o : normal);
-        va[J] = S::ld_c(P, ka, low ? a.off[io] : a.off[i]);
-        vb[J] = S::ld_c(P, io * N + o, a.off[io]);
-      }
-    });
-    sfor<CH>([&](auto J) {
-      constexpr int i = k0 + int(J);
-      if constexpr (i < Q) {
-        constexpr int io = opp(i);
-        const R qo = qv[io];
-        R d = R(0);
-        if (qo >= R(0.5)) {
-          const R inv2q = R(1) / (R(2) * fmin(qo, R(1)));
-          s[i] = inv2q * vb[J] + (R(1) - inv2q) * va[J];
-          if (qo <= R(1)) d = -(vb[J] - va[J]) / (R(2) * qo * qo);
-        } else if (qo >= R(0)) {
-          const R twoq = R(2) * qo;
-          s[i] = twoq * vb[J] + (R(1) - twoq) * va[J];
-          d = R(2) * (vb[J] - va[J]);
-        } else {
-          s[i] = va[J];
-        }
-        dsdq[i] = d;
-      }
-    });
-    __builtin_amdgcn_sched_barrier(0);
-  });
-}
-
 // The cotangent addition a link cell's Bouzidi pulls and force make to its
 // own entry P_k(x) (streaming.py:155-178, forces.py:95-118), from the
 // cotangents mu of its pulled values and the step's force seed g:
-//   a wall in direction opp(k) (q_k >= 0): c mu_opp(k) + e_k . g, with
+//   a wall in direction opp(k) (q_k = qk >= 0): c mu_opp(k) + e_k . g, with
 //     c = 1/(2 q_k) (q_k >= 1/2) or 2 q_k (below);
-//   a wall in direction k on the high branch (q_opp(k) >= 1/2):
+//   a wall in direction k on the high branch (q_opp(k) = qko >= 1/2):
 //     (1 - 1/(2 q_opp(k))) mu_k.
 // The remaining term, (1 - 2q) mu_i into P_opp(i)(x + e_i), goes to an entry
 // the cell owner-writes itself.  The coefficients are the forward's
 // (storage-precision q).
 template <int K, typename R>
-__device__ __forceinline__ R own_link_addition(const R* q, const R* mu, R gx, R gy, R gz) {
-  constexpr int ko = opp(K);
+__device__ __forceinline__ R own_link_addition(R qk, R qko, R muk, R muko, R gx, R gy, R gz) {
   R add = R(0);
-  const R qk = q[K];
   if (qk >= R(0)) {
     const R c = qk >= R(0.5) ? R(1) / (R(2) * fmin(qk, R(1))) : R(2) * qk;
     R f = R(0);
     add_scaled<EX[K]>(f, gx);
     add_scaled<EY[K]>(f, gy);
     add_scaled<EZ[K]>(f, gz);
-    add = fma(c, mu[ko], f);
+    add = fma(c, muko, f);
   }
-  const R qo = q[ko];
-  if (qo >= R(0.5)) add = fma(R(1) - R(1) / (R(2) * fmin(qo, R(1))), mu[K], add);
+  if (qko >= R(0.5)) add = fma(R(1) - R(1) / (R(2) * fmin(qko, R(1))), muk, add);
   return add;
 }
 
@@ -349,94 +295,224 @@ __device__ __forceinline__ void cell_transpose(const AdjArgs<typename S::R>& a, 
   }
 }
 
-// One cell: forward recompute + cotangent pull-back + owner writes.
-// MODE FULL: lam_in = dL/dP_{t+1}; MODE STREAM: lam_in = dL/dS_{t+1}.
-template <class S, class LM, int COLL, int MODE, bool LINK>
+// One dense-sweep cell: forward recompute + cotangent pull-back + owner
+// writes.  MODE FULL: lam_in = dL/dP_{t+1}; MODE STREAM: lam_in = dL/dS_{t+1}.
+// A thread on a link cell computes but writes nothing (the cell belongs to a
+// link workgroup).
+template <class S, class LM, int COLL, int MODE>
 __device__ __forceinline__ void adjoint_cell(const AdjArgs<typename S::R>& a, const typename S::T* __restrict__ P,
                                              const typename LM::T* __restrict__ lam_in,
-                                             typename LM::T* __restrict__ lam_out, const double* __restrict__ gF,
-                                             uint32_t c, int b, uint32_t fl, const typename S::R* qv,
-                                             typename S::R* dsdq) {
+                                             typename LM::T* __restrict__ lam_out, uint32_t c, uint32_t fl) {
   using R = typename S::R;
   const size_t N = a.P;
-  // a dense-sweep thread on a link cell computes but writes nothing (the
-  // cell belongs to a link workgroup)
-  const bool skip = !LINK && (fl & 2u);
+  const bool skip = fl & 2u;
   int x, y, z;
   adj_coords(a, c, x, y, z);
   const uint32_t o = adj_addr(a, x, y, z);
-  R s[Q];
-  uint32_t lmask = 0;  // bit j: direction j is a wall link (q_j >= 0)
-  if constexpr (LINK) {
-    pull_link<S, 6>(a, P, o, x, y, z, qv, s, dsdq);
-    sfor<Q>([&](auto J) {
-      constexpr int j = J;
-      if constexpr (j != 0) lmask |= (qv[j] >= R(0)) ? (1u << j) : 0u;
-    });
-    // qv lives in LDS: reload it for the owner writes instead of holding 18
-    // registers through the transposed collision
-    asm volatile("" ::: "memory");
-  } else {
-    pull_dense<S>(a, P, o, x, y, z, s);
-  }
-  R g[Q];
+  R s[Q], g[Q];
+  pull_dense<S>(a, P, o, x, y, z, s);
   auto load_g = [&](R* gg) {
     sfor<Q>([&](auto I) {
       constexpr int i = I;
       gg[i] = LM::ld(lam_in, i * N + o, a.lam_inv);
     });
-    if constexpr (LINK) {
-      if (a.add_prev) {
-        // the previous launch's additions to this cell's own entries, rounded
-        // to the cotangent storage as lbm_adjoint_link_finish would store them
-        sfor<Q>([&](auto I) {
-          constexpr int i = I;
-          if constexpr (i != 0)
-            gg[i] = LM::rt(gg[i] + a.add_prev[size_t(i) * a.n_links + b], a.lam_scale, a.lam_inv);
-        });
-      }
-    }
   };
-  cell_transpose<S, COLL, MODE, LINK>(a, s, g, R(1), load_g, c, x, fl, skip, lmask, gF);
-  // owner writes and Bouzidi bookkeeping
+  cell_transpose<S, COLL, MODE, false>(a, s, g, R(1), load_g, c, x, fl, skip, 0u, nullptr);
   if (skip) return;
   LM::st(lam_out, o, g[0], a.lam_scale);
+  // the owner targets are the pull sources: recompute them here rather than
+  // keep 18 addresses alive (or spilled) through the transposed collision
+  int xw = x, yw = y, zw = z;
+  asm volatile("" : "+v"(xw), "+v"(yw), "+v"(zw));
   sfor<Q>([&](auto I) {
     constexpr int i = I;
     if constexpr (i != 0) {
-      constexpr int io = opp(i);
-      const bool link = LINK && (lmask & (1u << io));
-      const bool normal = !link && !x_fallback<i>(x, a.nx);
-      R v = normal ? g[i] : R(0);
-      if constexpr (LINK) {
-        // folded: entry (x - e_i, i) is also the low-branch second neighbour
-        // of the wall direction opp(i) (q_i in [0, 1/2)): (1 - 2 q_i) mu_opp(i)
-        if (a.fold && qv[i] >= R(0) && qv[i] < R(0.5)) v = fma(R(1) - R(2) * qv[i], g[io], v);
-      }
-      LM::st(lam_out, i * N + owner_target<i>(a, x, y, z), v, a.lam_scale);
-      if constexpr (LINK) {
-        if (!a.fold) a.mu_link[size_t(i) * a.n_links + b] = link ? g[i] : R(0);
-        // d S_i / d q_opp(i) on the taken branch (from the pull)
-        if (link && a.dq_acc) a.dq_acc[size_t(io - 1) * a.n_links + b] += double(g[i]) * double(dsdq[i]);
-      }
+      const R v = x_fallback<i>(xw, a.nx) ? R(0) : g[i];
+      LM::st(lam_out, i * N + owner_target<i>(a, xw, yw, zw), v, a.lam_scale);
     }
   });
-  if constexpr (LINK) {
-    if (a.fold) {
-      // this step's additions to the cell's own entries, for the next launch
-      // (or lbm_adjoint_link_finish) to add to what it loads; one entry at a
-      // time, stored at once, so the link path keeps its register budget
-      R gx = R(0), gy = R(0), gz = R(0);
-      if (gF) {
-        gx = R(gF[0]);
-        gy = R(gF[1]);
-        gz = R(gF[2]);
-      }
-      sfor<Q>([&](auto K) {
-        constexpr int k = K;
-        if constexpr (k != 0) a.mu_link[size_t(k) * a.n_links + b] = own_link_addition<k>(qv, g, gx, gy, gz);
-      });
+}
+
+// LDS of an adjoint link workgroup: per direction and link cell (lane), the
+// pulled value S_i (then the cotangent mu_i), dS_i/dq_opp(i), q_i, and the
+// loaded cotangent of the cell's own P_i.
+template <typename R>
+struct LinkLds {
+  R s[Q][kLinkCellsPerBlock];
+  R dsdq[Q][kLinkCellsPerBlock];
+  R q[Q][kLinkCellsPerBlock];
+  R g[Q][kLinkCellsPerBlock];
+};
+
+// Phase 1 of an adjoint link workgroup, on the wave that owns the directions
+// i with i % NW == W: the forward's Bouzidi pull of S_i (streaming.py:113-181)
+// and dS_i/dq_opp(i) on the taken branch (0 outside the clip bounds), q_opp(i),
+// and the cotangent of the cell's own P_i with the previous launch's folded
+// addition (rounded to the cotangent storage as lbm_adjoint_link_finish would
+// store it).  Straight-line code over the wave's few directions: two rounds
+// of loads (q, then the populations and cotangents).  Sets bit opp(i) of mask
+// where opp(i) is a wall link.
+template <class S, class LM, int W, int NW>
+__device__ __forceinline__ void adj_link_pull_part(const AdjArgs<typename S::R>& a, const typename S::T* __restrict__ P,
+                                                   const typename LM::T* __restrict__ lam_in, int b, int x, int y,
+                                                   int z, uint32_t o, LinkLds<typename S::R>& L, int lane,
+                                                   uint32_t& mask) {
+  using R = typename S::R;
+  constexpr int ND = (Q - W + NW - 1) / NW;  // directions W, W + NW, ...
+  const size_t N = a.P;
+  R qo[ND], va[ND], vb[ND], lg[ND];
+  sfor<ND>([&](auto K) {
+    constexpr int i = W + int(K) * NW;
+    if constexpr (i != 0) qo[K] = R(a.link_q[size_t(opp(i) - 1) * a.n_links + b]);
+  });
+  sfor<ND>([&](auto K) {
+    constexpr int i = W + int(K) * NW;
+    lg[K] = LM::ld(lam_in, i * N + o, a.lam_inv);
+    if constexpr (i == 0) {
+      va[K] = S::ld_c(P, o, a.off[0]);
+    } else {
+      constexpr int io = opp(i);
+      const bool low = (qo[K] >= R(0)) & (qo[K] < R(0.5));
+      const bool high = qo[K] >= R(0.5);
+      const size_t normal = i * N + (x_fallback<i>(x, a.nx) ? o : owner_target<i>(a, x, y, z));
+      const size_t ka = low ? io * N + next_cell<i>(a, x, y, z) : (high ? i * N + o : normal);
+      va[K] = S::ld_c(P, ka, low ? a.off[io] : a.off[i]);
+      vb[K] = S::ld_c(P, io * N + o, a.off[io]);
+      if (a.add_prev) lg[K] = LM::rt(lg[K] + a.add_prev[size_t(i) * a.n_links + b], a.lam_scale, a.lam_inv);
     }
+  });
+  sfor<ND>([&](auto K) {
+    constexpr int i = W + int(K) * NW;
+    L.g[i][lane] = lg[K];
+    if constexpr (i == 0) {
+      L.s[0][lane] = va[K];
+      L.dsdq[0][lane] = R(0);
+      L.q[0][lane] = R(-1);
+    } else {
+      constexpr int io = opp(i);
+      const R q = qo[K];
+      R si, d = R(0);
+      if (q >= R(0.5)) {
+        const R inv2q = R(1) / (R(2) * fmin(q, R(1)));
+        si = inv2q * vb[K] + (R(1) - inv2q) * va[K];
+        if (q <= R(1)) d = -(vb[K] - va[K]) / (R(2) * q * q);
+      } else if (q >= R(0)) {
+        const R twoq = R(2) * q;
+        si = twoq * vb[K] + (R(1) - twoq) * va[K];
+        d = R(2) * (vb[K] - va[K]);
+      } else {
+        si = va[K];
+      }
+      L.s[i][lane] = si;
+      L.dsdq[i][lane] = d;
+      L.q[io][lane] = q;
+      if (q >= R(0)) mask |= 1u << io;
+    }
+  });
+}
+
+// Phase 3 of an adjoint link workgroup, on the wave that owns the directions
+// i with i % NW == W (i != 0), from the cotangents mu of the cell's pulled
+// values: the owner write of entry (x - e_i, i) -- mu_i for a normal pull, 0
+// for a wall link or an x face, plus, folded, the low-branch second-neighbour
+// term (1 - 2 q_i) mu_opp(i) of the wall direction opp(i) -- then dL/dq of the
+// wall link opp(i) and mu_link (the link cotangent, or folded, the addition
+// to the cell's own P_i for the next launch).
+template <class S, class LM, int W, int NW>
+__device__ __forceinline__ void adj_link_write_part(const AdjArgs<typename S::R>& a,
+                                                    typename LM::T* __restrict__ lam_out,
+                                                    const double* __restrict__ gF, int b, int x, int y, int z,
+                                                    uint32_t mask, const LinkLds<typename S::R>& L, int lane) {
+  using R = typename S::R;
+  constexpr int ND = (Q - W + NW - 1) / NW;
+  const size_t N = a.P;
+  R gx = R(0), gy = R(0), gz = R(0);
+  if (a.fold && gF) {
+    gx = R(gF[0]);
+    gy = R(gF[1]);
+    gz = R(gF[2]);
+  }
+  sfor<ND>([&](auto K) {
+    constexpr int i = W + int(K) * NW;
+    if constexpr (i != 0) {
+      constexpr int io = opp(i);
+      const R gi = L.g[i][lane], gio = L.g[io][lane];
+      const R qi = L.q[i][lane];
+      const bool link = mask & (1u << io);
+      const bool normal = !link && !x_fallback<i>(x, a.nx);
+      R v = normal ? gi : R(0);
+      if (a.fold && qi >= R(0) && qi < R(0.5)) v = fma(R(1) - R(2) * qi, gio, v);
+      LM::st(lam_out, i * N + owner_target<i>(a, x, y, z), v, a.lam_scale);
+      if (link && a.dq_acc) a.dq_acc[size_t(io - 1) * a.n_links + b] += double(gi) * double(L.dsdq[i][lane]);
+      a.mu_link[size_t(i) * a.n_links + b] =
+          a.fold ? own_link_addition<i>(qi, L.q[io][lane], gi, gio, gx, gy, gz) : (link ? gi : R(0));
+    }
+  });
+}
+
+// An adjoint link workgroup: kLinkCellsPerBlock link cells, lane l of every
+// wave on link b = first + l.  Phase 1 (all waves): the Bouzidi pulls and the
+// cotangent loads, spread over the waves by direction (adj_link_pull_part).
+// Phase 2 (wave 0, one lane per cell): Zou-He, the transposed collision, the
+// force seed and the Zou-He transposes (cell_transpose), leaving mu in LDS.
+// Phase 3 (all waves): the owner writes and link bookkeeping by direction
+// (adj_link_write_part).  A link cell so waits two rounds of loads before its
+// transposed collision, and no wave runs all 18 directions' address work.
+template <class S, class LM, int COLL, int MODE, int BS>
+__device__ __forceinline__ void adjoint_link_block(const AdjArgs<typename S::R>& a,
+                                                   const typename S::T* __restrict__ P,
+                                                   const typename LM::T* __restrict__ lam_in,
+                                                   typename LM::T* __restrict__ lam_out,
+                                                   const double* __restrict__ gF, uint32_t bid) {
+  using R = typename S::R;
+  constexpr int NW = BS / 64;
+  static_assert(kLinkCellsPerBlock == 64, "one link cell per lane");
+  __shared__ LinkLds<R> L;
+  __shared__ uint32_t lm[NW][kLinkCellsPerBlock];
+  const int lane = int(threadIdx.x & 63u);
+  const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+  const int b = int(bid) * kLinkCellsPerBlock + lane;
+  const bool live = b < a.n_links;
+  uint32_t mask = 0, c = 0, fl = 0, o = 0;
+  int x = 0, y = 0, z = 0;
+  if (live) {
+    c = uint32_t(a.link_cell[b]);
+    adj_coords(a, c, x, y, z);
+    o = adj_addr(a, x, y, z);
+    if (w == 0) fl = a.flags[c];
+    sfor<NW>([&](auto W) {
+      if (w == int(W)) adj_link_pull_part<S, LM, int(W), NW>(a, P, lam_in, b, x, y, z, o, L, lane, mask);
+    });
+  }
+  lm[w][lane] = mask;
+  __syncthreads();
+  mask = 0;
+#pragma unroll
+  for (int k = 0; k < NW; ++k) mask |= lm[k][lane];
+  if (w == 0 && live) {
+    R s[Q], g[Q];
+    sfor<Q>([&](auto I) {
+      constexpr int i = I;
+      s[i] = L.s[i][lane];
+    });
+    auto load_g = [&](R* gg) {
+      sfor<Q>([&](auto I) {
+        constexpr int i = I;
+        gg[i] = L.g[i][lane];
+      });
+    };
+    cell_transpose<S, COLL, MODE, true>(a, s, g, R(1), load_g, c, x, fl, false, mask, gF);
+    LM::st(lam_out, o, g[0], a.lam_scale);
+    sfor<Q>([&](auto I) {
+      constexpr int i = I;
+      L.g[i][lane] = g[i];
+    });
+  }
+  __syncthreads();
+  if (live) {
+    sfor<NW>([&](auto W) {
+      if (w == int(W)) adj_link_write_part<S, LM, int(W), NW>(a, lam_out, gF, b, x, y, z, mask, L, lane);
+    });
   }
 }
 
@@ -504,8 +580,8 @@ __device__ __forceinline__ void adjoint_row(const AdjArgs<typename S::R>& a, con
 }
 
 // Occupancy target of the dense adjoint sweep: the FP32 MRT transpose fits
-// 128 VGPRs (4 waves per SIMD) with a few spills confined to the inlet plane;
-// the other variants keep the compiler's allocation.
+// 128 VGPRs (4 waves per SIMD) without spills; the other variants keep the
+// compiler's allocation.
 template <class S, int COLL>
 constexpr int adj_waves_per_eu() {
   return (sizeof(typename S::R) == 4 && COLL == LBM_COLL_MRT) ? 4 : 1;
@@ -521,20 +597,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(adj_waves_pe
   using R = typename S::R;
   const uint32_t bid = blockIdx.x;
   if (int(bid) < a.nlb) {
-    // per-thread q and dS/dq of the link directions, kept out of the
-    // registers while the collision is transposed
-    __shared__ R qsh[kLinkCellsPerBlock][Q];
-    __shared__ R dsdq[kLinkCellsPerBlock][Q];
-    const int b = int(bid) * kLinkCellsPerBlock + int(threadIdx.x);
-    if (int(threadIdx.x) >= kLinkCellsPerBlock || b >= a.n_links) return;
-    const uint32_t c = uint32_t(a.link_cell[b]);
-    R* qv = qsh[threadIdx.x];
-    qv[0] = R(-1);
-    sfor<Q>([&](auto I) {
-      constexpr int i = I;
-      if constexpr (i != 0) qv[i] = R(a.link_q[size_t(i - 1) * a.n_links + b]);
-    });
-    adjoint_cell<S, LM, COLL, MODE, true>(a, P, lam_in, lam_out, gF, c, b, a.flags[c], qv, dsdq[threadIdx.x]);
+    adjoint_link_block<S, LM, COLL, MODE, BS>(a, P, lam_in, lam_out, gF, bid);
     return;
   }
   // tiles in workgroup order: consecutive rows run on all XCDs at once
@@ -545,8 +608,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(adj_waves_pe
     adjoint_row<S, LM, COLL, MODE>(a, P, lam_in, lam_out, c);
     return;
   }
-  adjoint_cell<S, LM, COLL, MODE, false>(a, P, lam_in, lam_out, gF, c, -1, a.flags ? uint32_t(a.flags[c]) : 0u,
-                                         nullptr, nullptr);
+  adjoint_cell<S, LM, COLL, MODE>(a, P, lam_in, lam_out, c, a.flags ? uint32_t(a.flags[c]) : 0u);
 }
 
 // Prologue transpose: dL/dS_0 = ([pre guard] collide bounce)^T lambda_0, cell-local.
