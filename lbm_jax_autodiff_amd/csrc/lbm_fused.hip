@@ -352,14 +352,15 @@ constexpr int step_waves_per_eu() {
 // neighbour P(x + e_i)_opp below q = 1/2, the cell's own P_i above, the
 // normal pull where opp(i) is no wall link), forms S_i into LDS, adds the own
 // part e_opp(i) P_opp(i) of the momentum exchange (forces.py:95-118) to f and
-// sets bit opp(i) of mask where opp(i) is a wall link.  Straight-line code
-// over the wave's few directions, so its loads go out in two rounds (q, then
-// the populations).
+// sets bit opp(i) of mask where opp(i) is a wall link.  With `pulled` (a cell
+// that neither Zou-He nor the post guard changes) it also adds the pulled
+// part e_opp(i) S_i.  Straight-line code over the wave's few directions, so
+// its loads go out in two rounds (q, then the populations).
 template <class S, int W, int NW>
 __device__ __forceinline__ void link_pull_part(const StepArgs<typename S::R>& a, const typename S::T* __restrict__ src,
                                                int b, int x, int y, int z, uint32_t o, const Rows& rows,
-                                               typename S::R (*sl)[kLinkCellsPerBlock], int lane, double* f,
-                                               uint32_t& mask) {
+                                               typename S::R (*sl)[kLinkCellsPerBlock], int lane, bool pulled,
+                                               double* f, uint32_t& mask) {
   using R = typename S::R;
   constexpr int ND = (Q - W + NW - 1) / NW;  // directions W, W + NW, ...
   const size_t N = a.P;
@@ -406,16 +407,23 @@ __device__ __forceinline__ void link_pull_part(const StepArgs<typename S::R>& a,
         add_scaled<EX[io]>(f[0], t);
         add_scaled<EY[io]>(f[1], t);
         add_scaled<EZ[io]>(f[2], t);
+        if (pulled) {
+          const double u = double(si);
+          add_scaled<EX[io]>(f[0], u);
+          add_scaled<EY[io]>(f[1], u);
+          add_scaled<EZ[io]>(f[2], u);
+        }
       }
     }
   });
 }
 
 // A link workgroup: kLinkCellsPerBlock link cells, lane l of every wave on
-// link b = first + l.  Phase 1 (all waves): the Bouzidi pulls, spread over
-// the waves by direction (link_pull_part).  Phase 2 (wave 0): per cell,
-// Zou-He, the pulled part e_j S_opp(j) of the momentum exchange, collision
-// and store.  The workgroup's force partial (float64) goes to its slot of
+// link b = first + l.  Phase 1 (all waves): the Bouzidi pulls and the
+// momentum exchange, spread over the waves by direction (link_pull_part).
+// Phase 2 (wave 0): per cell, Zou-He, the pulled part e_j S_opp(j) of the
+// momentum exchange where Zou-He or the post guard changed S, collision and
+// store.  The workgroup's force partial (float64) goes to its slot of
 // fpart.  A link cell so waits two round trips before its collision, and no
 // wave runs all 18 directions' address and branch work.
 template <class S, int COLL, int MODE, int BS>
@@ -441,8 +449,9 @@ __device__ __forceinline__ void link_block(const StepArgs<typename S::R>& a, con
     o = cell_addr(a, x, y, z);
     if (w == 0 && MODE == LBM_MODE_FULL) fl = a.flags[c];
     const Rows rows = make_rows(a, y, z);
+    const bool pulled = x != 0 && x != a.nx - 1 && a.guard != LBM_GUARD_POST;
     sfor<NW>([&](auto W) {
-      if (w == int(W)) link_pull_part<S, int(W), NW>(a, src, b, x, y, z, o, rows, sl, lane, f, mask);
+      if (w == int(W)) link_pull_part<S, int(W), NW>(a, src, b, x, y, z, o, rows, sl, lane, pulled, f, mask);
     });
   }
   lm[w][lane] = mask;
@@ -458,14 +467,17 @@ __device__ __forceinline__ void link_block(const StepArgs<typename S::R>& a, con
   if (w != 0) return;
   R s[Q];
   double g[3] = {0.0, 0.0, 0.0};
+  const bool face = live && (x == 0 || x == a.nx - 1 || a.guard == LBM_GUARD_POST);
   if (live) {
-#pragma unroll
-    for (int k = 1; k < NW; ++k) mask |= lm[k][lane];
     sfor<Q>([&](auto I) {
       constexpr int i = I;
       s[i] = sl[i][lane];
     });
     boundaries(a, s, x);
+  }
+  if (face) {
+#pragma unroll
+    for (int k = 1; k < NW; ++k) mask |= lm[k][lane];
     sfor<Q>([&](auto J) {
       constexpr int j = J;
       if constexpr (j != 0) {
@@ -478,9 +490,9 @@ __device__ __forceinline__ void link_block(const StepArgs<typename S::R>& a, con
       }
     });
   }
-  // the pulled part joins wave 0's partial before the collision, which needs
-  // every register the dense budget leaves
-  if (fpart) {
+  // the pulled part of the x-face cells joins wave 0's partial before the
+  // collision, which needs every register the dense budget leaves
+  if (fpart && __any(face)) {
     const double gx = wave_sum(g[0]), gy = wave_sum(g[1]), gz = wave_sum(g[2]);
     if (lane == 0) {
       red[0][0] += gx;
