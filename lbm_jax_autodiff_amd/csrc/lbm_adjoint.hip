@@ -188,22 +188,26 @@ __device__ __forceinline__ void apply_zou_he(const AdjArgs<R>& a, R* s, int x) {
 // g <- (pre-guard, collide, bounce)^T g at the collision input s (the cell's
 // S_{t+1}); s is replaced by the pre-guard reset state where the guard fires.
 // Accumulates dL/dporosity (the term times gunit, g's unit) unless `skip`.
-// fl = the cell's flags.  Returns true when the pre guard fired.
-template <typename R, int COLL>
+// fl = the cell's flags.  Returns true when the pre guard fired.  SOFT: 1
+// the caller knows the bounce is soft, 0 that it is not, -1 neither (read at
+// run time); a sweep compiled for one case keeps the other branch out of its
+// code (the soft path's new registers would make every other cell copy its
+// 19 cotangents at the branch join).
+template <bool RCP, int COLL, int SOFT = -1, typename R>
 __device__ __forceinline__ bool collide_transpose(const AdjArgs<R>& a, R* s, R* g, R gunit, uint32_t c, uint32_t fl,
                                                   bool skip) {
   Pairs<R> ps = to_pairs(s);
-  Macro<R> m = macro(ps);
+  Macro<R> m = macro<RCP>(ps);
   const bool pre_reset = (a.guard == LBM_GUARD_PRE) && guard_bad(m);
   if (pre_reset) {
     set_feq_unit(s, a.uref[0], a.uref[1], a.uref[2]);
     ps = to_pairs(s);
-    m = macro(ps);
+    m = macro<RCP>(ps);
   }
   const R omega = a.smag ? R(1) / smag_tau(ps, m, a.tau, a.cs) : a.omega;
-  if (a.bounce == LBM_BOUNCE_HARD) {
+  if (SOFT != 1 && a.bounce == LBM_BOUNCE_HARD) {
     if (fl & 1) reverse_all(g);
-  } else if (a.bounce == LBM_BOUNCE_SOFT) {
+  } else if (SOFT != 0 && a.bounce == LBM_BOUNCE_SOFT) {
     R cc[Q];
     sfor<Q>([&](auto I) {
       constexpr int i = I;
@@ -214,22 +218,24 @@ __device__ __forceinline__ bool collide_transpose(const AdjArgs<R>& a, R* s, R* 
     const R gp = blend_vjp(cc, g, a.phi[c]);
     if (a.phi_acc && !skip) a.phi_acc[c] += double(gp) * double(gunit);
   }
+  // dL/domega is needed only for the Smagorinsky path (a uniform branch;
+  // without it the transpose skips the stress moments' dm).  A cell the pre
+  // guard reset passes no cotangent back: it is transposed like the others
+  // (at its reset state) and then cleared, so the cleared case writes zeros
+  // into the same registers instead of making every cell copy at a join.
+  if (a.smag) {
+    const R gom = (COLL == LBM_COLL_MRT) ? mrt_vjp<true>(ps, m, g, omega) : bgk_vjp<true>(ps, m, g, omega);
+    smag_vjp(ps, m, g, a.tau, a.cs, -gom * omega * omega);
+  } else if constexpr (COLL == LBM_COLL_MRT) {
+    mrt_vjp<false>(ps, m, g, omega);
+  } else {
+    bgk_vjp<false>(ps, m, g, omega);
+  }
   if (pre_reset) {
     sfor<Q>([&](auto I) {
       constexpr int i = I;
       g[i] = R(0);
     });
-  } else {
-    // dL/domega is needed only for the Smagorinsky path (a uniform branch;
-    // without it the transpose skips the stress moments' dm)
-    if (a.smag) {
-      const R gom = (COLL == LBM_COLL_MRT) ? mrt_vjp<true>(ps, m, g, omega) : bgk_vjp<true>(ps, m, g, omega);
-      smag_vjp(ps, m, g, a.tau, a.cs, -gom * omega * omega);
-    } else if constexpr (COLL == LBM_COLL_MRT) {
-      mrt_vjp<false>(ps, m, g, omega);
-    } else {
-      bgk_vjp<false>(ps, m, g, omega);
-    }
   }
   return pre_reset;
 }
@@ -243,8 +249,8 @@ __device__ __forceinline__ bool collide_transpose(const AdjArgs<R>& a, R* s, R* 
 // out of the registers while those are computed.  g may hold the cotangent
 // times a constant (a storage scale): the map is linear in g, and gunit
 // (1 / that constant) converts the accumulated parameter terms; the force
-// seed (link cells) needs gunit = 1.
-template <class S, int COLL, int MODE, bool LINK, class LoadG>
+// seed (link cells) needs gunit = 1.  SOFT as for collide_transpose.
+template <class S, int COLL, int MODE, bool LINK, int SOFT = -1, class LoadG>
 __device__ __forceinline__ void cell_transpose(const AdjArgs<typename S::R>& a, typename S::R* s, typename S::R* g,
                                                typename S::R gunit, LoadG&& load_g, uint32_t c, int x, uint32_t fl,
                                                bool skip, uint32_t lmask, const double* __restrict__ gF) {
@@ -253,10 +259,15 @@ __device__ __forceinline__ void cell_transpose(const AdjArgs<typename S::R>& a, 
   R rho_in = R(0);
   if (x == 0) rho_in = zou_he_in_rho(s, a.uin[0]);
   apply_zou_he(a, s, x);
-  const bool post_reset = (a.guard == LBM_GUARD_POST) && guard_bad(s);
-  if (post_reset) set_weights(s);
+  // the reset stays under the uniform guard-mode test (if-converted outside
+  // it, every cell would select its 19 values)
+  bool post_reset = false;
+  if (a.guard == LBM_GUARD_POST) {
+    post_reset = guard_bad<rcp_of<S>>(s);
+    if (post_reset) set_weights(s);
+  }
   load_g(g);
-  if constexpr (MODE == LBM_MODE_FULL) collide_transpose<R, COLL>(a, s, g, gunit, c, fl, skip);
+  if constexpr (MODE == LBM_MODE_FULL) collide_transpose<rcp_of<S>, COLL, SOFT>(a, s, g, gunit, c, fl, skip);
   // momentum-exchange force uses S_{t+1}(x)_opp(j) for links j (forces.py:95-118)
   if constexpr (LINK) {
     if (gF) {
@@ -525,7 +536,7 @@ __device__ __forceinline__ void adjoint_link_block(const AdjArgs<typename S::R>&
 // neighbour instead of the cell's own value: Zou-He overwrites exactly those
 // populations from the others (boundary.py:113-122, 190-198), so the recomputed
 // step is the same.
-template <class S, class LM, int COLL, int MODE>
+template <class S, class LM, int COLL, int MODE, int SOFT>
 __device__ __forceinline__ void adjoint_row(const AdjArgs<typename S::R>& a, const typename S::T* __restrict__ P,
                                             const typename LM::T* __restrict__ lam_in,
                                             typename LM::T* __restrict__ lam_out, uint32_t c) {
@@ -580,7 +591,7 @@ __device__ __forceinline__ void adjoint_row(const AdjArgs<typename S::R>& a, con
   };
   // g stays in the stored units (lambda times the storage scale, a power of
   // two): the transpose commutes with it exactly
-  cell_transpose<S, COLL, MODE, false>(a, s, g, a.lam_inv, load_g, c, x, fl, skip, 0u, nullptr);
+  cell_transpose<S, COLL, MODE, false, SOFT>(a, s, g, a.lam_inv, load_g, c, x, fl, skip, 0u, nullptr);
   if (skip) return;
   sfor<Q>([&](auto I) {
     constexpr int i = I;
@@ -618,7 +629,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(adj_waves_pe
   const uint32_t c = tile * BS + threadIdx.x;
   if (c >= a.N) return;
   if constexpr (ROW) {
-    adjoint_row<S, LM, COLL, MODE>(a, P, lam_in, lam_out, c);
+    if (a.bounce == LBM_BOUNCE_SOFT) adjoint_row<S, LM, COLL, MODE, 1>(a, P, lam_in, lam_out, c);
+    else adjoint_row<S, LM, COLL, MODE, 0>(a, P, lam_in, lam_out, c);
     return;
   }
   adjoint_cell<S, LM, COLL, MODE>(a, P, lam_in, lam_out, c, a.flags ? uint32_t(a.flags[c]) : 0u);
@@ -643,7 +655,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjoint_collide(AdjArgs<typename 
     s[i] = S::ld_c(S0, i * N + o, a.off[i]);
     g[i] = LM::ld(lam_in, i * N + o, a.lam_inv);
   });
-  collide_transpose<R, COLL>(a, s, g, R(1), c, a.flags ? uint32_t(a.flags[c]) : 0u, false);
+  collide_transpose<rcp_of<S>, COLL>(a, s, g, R(1), c, a.flags ? uint32_t(a.flags[c]) : 0u, false);
   sfor<Q>([&](auto I) {
     constexpr int i = I;
     g_out[i * N + o] = g[i];

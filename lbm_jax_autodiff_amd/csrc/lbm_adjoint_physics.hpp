@@ -17,7 +17,7 @@ namespace lbm {
 // u = j / rho:   g_i += G_rho - (G_u . u)/rho + (G_u . e_i)/rho.
 template <typename R>
 __device__ __forceinline__ void add_macro_pullback(R* g, const Macro<R>& m, R Grho, R Gux, R Guy, R Guz) {
-  const R inv = R(1) / m.rho;
+  const R inv = m.inv;
   const R a = Grho - (Gux * m.ux + Guy * m.uy + Guz * m.uz) * inv;
   const R bx = Gux * inv, by = Guy * inv, bz = Guz * inv;
   sfor<Q>([&](auto I) {

@@ -137,6 +137,20 @@ __device__ __forceinline__ unsigned short float_to_bf16_bits(float d) {
 }
 __device__ __forceinline__ float f16_bits_to_float(unsigned short raw) { return __half2float(__ushort_as_half(raw)); }
 __device__ __forceinline__ unsigned short float_to_f16_bits(float d) { return __half_as_ushort(__float2half_rn(d)); }
+// float(h) + off for binary16 bits h, in one v_fma_mix_f32 (op_sel_hi marks
+// the half operand).  `off` must be wave-uniform (it is read from an SGPR).
+__device__ __forceinline__ float f16_bits_add(unsigned short raw, float off) {
+  float r;
+  asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(uint32_t(raw)), "s"(off));
+  return r;
+}
+// The binary16 bits of v - off, rounded once to nearest even, in one
+// v_fma_mixlo_f16.  `off` must be wave-uniform (it is read from an SGPR).
+__device__ __forceinline__ unsigned short f16_bits_of_diff(float v, float off) {
+  uint32_t r;
+  asm("v_fma_mixlo_f16 %0, %1, 1.0, -%2 op_sel_hi:[0,0,0]" : "=v"(r) : "v"(v), "s"(off));
+  return static_cast<unsigned short>(r);
+}
 
 struct StoreF32 {
   using T = float;
@@ -213,32 +227,45 @@ struct StoreF64 {
     if constexpr (V == 1) __builtin_nontemporal_store(o[0], p + k);
   }
 };
+// Every access of a half format takes a wave-uniform offset (the offset of a
+// compile-time population, or of a population uniform over the workgroup).
+// For binary16 the conversion and the offset fold into one mixed-precision
+// FMA each way: v_fma_mix_f32 (half operand * 1.0 + off, rounded once to
+// float) on load and v_fma_mixlo_f16 (v * 1.0 - off, rounded once to half,
+// nearest even) on store.  bfloat16 shifts the bits on load and rounds with
+// v_cvt_pk_bf16_f32 on store.
 template <bool BF>
 struct StoreHalfDev {
   using T = typename std::conditional<BF, __hip_bfloat16, __half>::type;
   using R = float;
   static constexpr int tag = BF ? LBM_BF16DEV : LBM_F16DEV;
-  __device__ static float dec(unsigned short raw) { return BF ? bf16_bits_to_float(raw) : f16_bits_to_float(raw); }
-  __device__ static unsigned short enc(float d) { return BF ? float_to_bf16_bits(d) : float_to_f16_bits(d); }
+  __device__ static float dec(unsigned short raw, float off) {
+    if constexpr (BF) return bf16_bits_to_float(raw) + off;
+    else return f16_bits_add(raw, off);
+  }
+  __device__ static unsigned short enc(float v, float off) {
+    if constexpr (BF) return float_to_bf16_bits(v - off);
+    else return f16_bits_of_diff(v, off);
+  }
   __device__ static const unsigned short* bits(const T* p) { return reinterpret_cast<const unsigned short*>(p); }
   __device__ static unsigned short* bits(T* p) { return reinterpret_cast<unsigned short*>(p); }
-  __device__ static R ld(const T* p, size_t k, R off) { return dec(__builtin_nontemporal_load(bits(p) + k)) + off; }
-  __device__ static R ld_c(const T* p, size_t k, R off) { return dec(bits(p)[k]) + off; }
-  __device__ static void st(T* p, size_t k, R v, R off) { __builtin_nontemporal_store(enc(v - off), bits(p) + k); }
+  __device__ static R ld(const T* p, size_t k, R off) { return dec(__builtin_nontemporal_load(bits(p) + k), off); }
+  __device__ static R ld_c(const T* p, size_t k, R off) { return dec(bits(p)[k], off); }
+  __device__ static void st(T* p, size_t k, R v, R off) { __builtin_nontemporal_store(enc(v, off), bits(p) + k); }
   __device__ static R bld(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, R off) {
-    return dec(__builtin_amdgcn_raw_buffer_load_b16(r, int(vo), int(so), 0)) + off;
+    return dec(__builtin_amdgcn_raw_buffer_load_b16(r, int(vo), int(so), 0), off);
   }
   __device__ static void bst(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, R v, R off) {
-    __builtin_amdgcn_raw_buffer_store_b16(enc(v - off), r, int(vo), int(so), kBufNT);
+    __builtin_amdgcn_raw_buffer_store_b16(enc(v, off), r, int(vo), int(so), kBufNT);
   }
   template <int V>
   __device__ static void ldv(const T* p, size_t k, R off, R* o) {
     if constexpr (V == 4) {
       const u16x4 v = *reinterpret_cast<const u16x4*>(bits(p) + k);
-      o[0] = dec(v[0]) + off; o[1] = dec(v[1]) + off; o[2] = dec(v[2]) + off; o[3] = dec(v[3]) + off;
+      o[0] = dec(v[0], off); o[1] = dec(v[1], off); o[2] = dec(v[2], off); o[3] = dec(v[3], off);
     } else if constexpr (V == 2) {
       const u16x2 v = *reinterpret_cast<const u16x2*>(bits(p) + k);
-      o[0] = dec(v[0]) + off; o[1] = dec(v[1]) + off;
+      o[0] = dec(v[0], off); o[1] = dec(v[1], off);
     } else {
       o[0] = ld_c(p, k, off);
     }
@@ -246,10 +273,10 @@ struct StoreHalfDev {
   template <int V>
   __device__ static void stv(T* p, size_t k, const R* o, R off) {
     if constexpr (V == 4) {
-      const u16x4 v = {enc(o[0] - off), enc(o[1] - off), enc(o[2] - off), enc(o[3] - off)};
+      const u16x4 v = {enc(o[0], off), enc(o[1], off), enc(o[2], off), enc(o[3], off)};
       __builtin_nontemporal_store(v, reinterpret_cast<u16x4*>(bits(p) + k));
     } else if constexpr (V == 2) {
-      const u16x2 v = {enc(o[0] - off), enc(o[1] - off)};
+      const u16x2 v = {enc(o[0], off), enc(o[1], off)};
       __builtin_nontemporal_store(v, reinterpret_cast<u16x2*>(bits(p) + k));
     } else {
       st(p, k, o[0], off);

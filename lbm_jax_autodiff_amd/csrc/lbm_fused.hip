@@ -138,23 +138,27 @@ __device__ __forceinline__ uint32_t wrap_next(const StepArgs<R>& a, int x, int y
 }
 
 // Zou-He planes and the post-collision guard on a pulled S (state.py:152-169).
-template <typename R>
+template <bool RCP, typename R>
 __device__ __forceinline__ void boundaries(const StepArgs<R>& a, R* s, int x) {
   if (x == 0) zou_he_in(s, a.uin[0], a.uin[1], a.uin[2], false, R(0));
   if (x == a.nx - 1) zou_he_out(s, a.rho_out);
-  if (a.guard == LBM_GUARD_POST && guard_bad(s)) set_weights(s);
+  if (a.guard == LBM_GUARD_POST && guard_bad<RCP>(s)) set_weights(s);
 }
 
 // [pre guard] -> [Smagorinsky tau] -> BGK|MRT -> hard|soft bounce
 // (examples/10:71-78, state.py:133-149, smooth.py:151-155).
-template <typename R, int COLL>
+// SOFT: 1 the caller knows the bounce is soft, 0 that it is not, -1 neither
+// (the mode is read at run time).  A sweep compiled for one case keeps the
+// other branch out of its code: a soft blend writing new registers would
+// otherwise make every hard-mask cell copy its 19 values at the branch join.
+template <bool RCP, int COLL, int SOFT = -1, typename R>
 __device__ __forceinline__ void collide_cell(const StepArgs<R>& a, R* s, uint32_t c, uint32_t fl) {
   bool reset;
-  relax<COLL == LBM_COLL_MRT>(s, a.tau, a.omega, a.guard == LBM_GUARD_PRE, a.uref[0], a.uref[1], a.uref[2],
-                              a.smag != 0, a.cs, reset);
-  if (a.bounce == LBM_BOUNCE_HARD) {
+  relax<COLL == LBM_COLL_MRT, RCP>(s, a.tau, a.omega, a.guard == LBM_GUARD_PRE, a.uref[0], a.uref[1], a.uref[2],
+                                   a.smag != 0, a.cs, reset);
+  if (SOFT != 1 && a.bounce == LBM_BOUNCE_HARD) {
     if (fl & 1) reverse_all(s);
-  } else if (a.bounce == LBM_BOUNCE_SOFT) {
+  } else if (SOFT != 0 && a.bounce == LBM_BOUNCE_SOFT) {
     blend_opposite(s, a.phi[c]);
   }
 }
@@ -249,11 +253,11 @@ __device__ __forceinline__ void dense_vec(const StepArgs<typename S::R>& a, cons
 #pragma unroll
   for (int j = 0; j < V; ++j) {
     const uint32_t fj = (fl >> (8 * j)) & 0xffu;
-    if constexpr (MODE != LBM_MODE_COLLIDE) boundaries(a, s[j], x0 + j);
+    if constexpr (MODE != LBM_MODE_COLLIDE) boundaries<rcp_of<S>>(a, s[j], x0 + j);
     if constexpr (MODE != LBM_MODE_STREAM) {
       bool reset;
-      relax<COLL == LBM_COLL_MRT>(s[j], a.tau, a.omega, a.guard == LBM_GUARD_PRE, a.uref[0], a.uref[1], a.uref[2],
-                                  a.smag != 0, a.cs, reset);
+      relax<COLL == LBM_COLL_MRT, rcp_of<S>>(s[j], a.tau, a.omega, a.guard == LBM_GUARD_PRE, a.uref[0], a.uref[1],
+                                             a.uref[2], a.smag != 0, a.cs, reset);
       if (a.bounce == LBM_BOUNCE_HARD) {
         if (fj & 1u) reverse_all(s[j]);
       } else if (a.bounce == LBM_BOUNCE_SOFT) {
@@ -288,8 +292,9 @@ __device__ __forceinline__ void dense_vec(const StepArgs<typename S::R>& a, cons
 // x = 0 the e_x = +1 pulls and at x = nx - 1 the e_x = -1 pulls read the
 // wrapped neighbour instead of the cell's own value: Zou-He then overwrites
 // exactly those populations from the others (boundary.py:113-122, 190-198),
-// so the step's result is the same.  PER: periodic y/z (no clamped rows).
-template <class S, int COLL, int MODE, bool PER>
+// so the step's result is the same.  PER: periodic y/z (no clamped rows);
+// SOFT as for collide_cell.
+template <class S, int COLL, int MODE, bool PER, int SOFT>
 __device__ __forceinline__ void dense_row(const StepArgs<typename S::R>& a, const typename S::T* __restrict__ src,
                                           typename S::T* __restrict__ dst, uint32_t c) {
   using R = typename S::R;
@@ -328,8 +333,8 @@ __device__ __forceinline__ void dense_row(const StepArgs<typename S::R>& a, cons
     }
     s[i] = S::bld(rin, bx, (uint32_t(i) * a.P + row) * B, a.off[i]);
   });
-  boundaries(a, s, x);
-  if (MODE == LBM_MODE_FULL) collide_cell<R, COLL>(a, s, c, fl);
+  boundaries<rcp_of<S>>(a, s, x);
+  if (MODE == LBM_MODE_FULL) collide_cell<rcp_of<S>, COLL, SOFT>(a, s, c, fl);
   if (fl & 2u) return;  // owned by a link workgroup
   const __amdgpu_buffer_rsrc_t rout = buffer_rsrc(dst, a.state_bytes);
   sfor<Q>([&](auto I) {
@@ -473,7 +478,7 @@ __device__ __forceinline__ void link_block(const StepArgs<typename S::R>& a, con
       constexpr int i = I;
       s[i] = sl[i][lane];
     });
-    boundaries(a, s, x);
+    boundaries<rcp_of<S>>(a, s, x);
   }
   if (face) {
 #pragma unroll
@@ -501,7 +506,7 @@ __device__ __forceinline__ void link_block(const StepArgs<typename S::R>& a, con
     }
   }
   if (live) {
-    if (MODE == LBM_MODE_FULL) collide_cell<R, COLL>(a, s, c, fl);
+    if (MODE == LBM_MODE_FULL) collide_cell<rcp_of<S>, COLL>(a, s, c, fl);
     store_cell<S>(a, dst, o, s);
   }
   if (fpart && lane < 3) {
@@ -535,8 +540,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(step_waves_p
   const uint32_t c = tile * BS + threadIdx.x;
   if (c >= a.N) return;
   if constexpr (V == 0) {
-    if (a.periodic) dense_row<S, COLL, MODE, true>(a, src, dst, c);
-    else dense_row<S, COLL, MODE, false>(a, src, dst, c);
+    if (!a.periodic) dense_row<S, COLL, MODE, false, -1>(a, src, dst, c);
+    else if (a.bounce == LBM_BOUNCE_SOFT) dense_row<S, COLL, MODE, true, 1>(a, src, dst, c);
+    else dense_row<S, COLL, MODE, true, 0>(a, src, dst, c);
     return;
   }
   // The flags are loaded together with the populations; testing the
@@ -551,7 +557,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(step_waves_p
       constexpr int i = I;
       s[i] = S::ld(src, i * N + o, a.off[i]);
     });
-    collide_cell<R, COLL>(a, s, c, fl);
+    collide_cell<rcp_of<S>, COLL>(a, s, c, fl);
   } else {
     int x, y, z;
     cell_coords(a, c, x, y, z);
@@ -561,8 +567,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(step_waves_p
       constexpr int i = I;
       s[i] = S::ld_c(src, i * N + pull_src<i>(a, rows, o, x), a.off[i]);
     });
-    boundaries(a, s, x);
-    if (MODE == LBM_MODE_FULL) collide_cell<R, COLL>(a, s, c, fl);
+    boundaries<rcp_of<S>>(a, s, x);
+    if (MODE == LBM_MODE_FULL) collide_cell<rcp_of<S>, COLL>(a, s, c, fl);
     if (fl & 2u) return;  // owned by a link workgroup
   }
   store_cell<S>(a, dst, o, s);
@@ -583,17 +589,20 @@ __global__ void k_force_reduce(const double* __restrict__ part, int T, int slots
   force[3 * t + 2] = fz;
 }
 
+// One population per grid row (blockIdx.y), so the offset a half format is
+// encoded against is uniform over the workgroup, as in the sweeps.
 template <class SI, class SO>
-__global__ void k_convert(const typename SI::T* __restrict__ in, typename SO::T* __restrict__ out, StepArgs<float> a,
-                          size_t total) {
-  const size_t k = blockIdx.x * size_t(blockDim.x) + threadIdx.x;
-  if (k >= total) return;
-  const int i = int(k / a.N);
-  const size_t e = size_t(i) * a.P + cell_addr(a, uint32_t(k - size_t(i) * a.N));
+__global__ void k_convert(const typename SI::T* __restrict__ in, typename SO::T* __restrict__ out,
+                          StepArgs<float> a) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.N) return;
+  const int i = int(blockIdx.y);
+  const size_t e = size_t(i) * a.P + cell_addr(a, c);
   using RI = typename SI::R;
   using RO = typename SO::R;
-  const RI v = SI::ld_c(in, e, RI(a.off[i]));
-  SO::st(out, e, RO(v), RO(a.off[i]));
+  const float off = a.off[i];
+  const RI v = SI::ld_c(in, e, RI(off));
+  SO::st(out, e, RO(v), RO(off));
 }
 
 template <typename R>
@@ -825,11 +834,10 @@ int lbm_convert_state(const lbm_step_desc* d, int src_dtype, const void* src, in
   const int rc = make_args(d, LBM_MODE_COLLIDE, a);
   if (rc) return rc;
   if (!src || !dst) return LBM_E_ARG;
-  const size_t total = size_t(a.N) * Q;
-  const unsigned grid = unsigned((total + 255) / 256);
+  const dim3 grid((a.N + 255) / 256, Q);
   hipStream_t s = (hipStream_t)stream;
-#define LBM_CONV(SI, SO) hipLaunchKernelGGL((k_convert<SI, SO>), grid, 256, 0, s, \
-                                            (const SI::T*)src, (SO::T*)dst, a, total)
+#define LBM_CONV(SI, SO) hipLaunchKernelGGL((k_convert<SI, SO>), grid, dim3(256), 0, s, \
+                                            (const SI::T*)src, (SO::T*)dst, a)
   if (src_dtype == LBM_F32 && dst_dtype == LBM_F16DEV) LBM_CONV(StoreF32, StoreF16);
   else if (src_dtype == LBM_F32 && dst_dtype == LBM_BF16DEV) LBM_CONV(StoreF32, StoreBF16);
   else if (src_dtype == LBM_F16DEV && dst_dtype == LBM_F32) LBM_CONV(StoreF16, StoreF32);

@@ -50,6 +50,7 @@ __global__ void k_equilibrium_vjp(const R* __restrict__ rho, const R* __restrict
   m.uy = u[N + c];
   m.uz = u[2 * N + c];
   m.usq = m.ux * m.ux + m.uy * m.uy + m.uz * m.uz;
+  m.inv = R(1) / m.rho;
   R g[Q];
   vload(gout, N, c, g);
   const Pairs<R> pg = to_pairs(g);

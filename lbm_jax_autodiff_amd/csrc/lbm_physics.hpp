@@ -17,10 +17,27 @@
 
 namespace lbm {
 
+// rho, u and u.u of a cell, with inv = 1 / rho kept for the adjoint's
+// pull-back through u = j / rho.
 template <typename R>
 struct Macro {
-  R rho, ux, uy, uz, usq;
+  R rho, ux, uy, uz, usq, inv;
 };
+
+// 1 / rho: IEEE division, or with RCP (float only) the hardware reciprocal
+// v_rcp_f32 (1 ulp), which the 16-bit storage sweeps use: their arithmetic is
+// on the critical path and their populations carry 11-bit mantissas.
+template <bool RCP, typename R>
+__device__ __forceinline__ R recip(R x) {
+  if constexpr (RCP && sizeof(R) == 4) return __builtin_amdgcn_rcpf(x);
+  else return R(1) / x;
+}
+
+// The fused step and its adjoint take 1 / rho from the hardware reciprocal
+// for the 16-bit storage formats (storage S), from IEEE division otherwise;
+// the adjoint's recompute of a step makes the same choice as the step.
+template <class S>
+constexpr bool rcp_of = sizeof(typename S::T) == 2;
 
 // acc += c * v for a compile-time integer c, without multiplications by +-1
 // and with nothing emitted for c == 0.
@@ -75,7 +92,7 @@ __device__ __forceinline__ Pairs<R> to_pairs(const R* f) {
 
 // rho = sum f; u = sum f e / rho (equilibrium.py:22-87), from the pairs:
 // opposite directions contribute e_a (f_a - f_opp) to the momentum.
-template <typename R>
+template <bool RCP = false, typename R>
 __device__ __forceinline__ Macro<R> macro(const Pairs<R>& p) {
   R rho = p.f0, jx = R(0), jy = R(0), jz = R(0);
   sfor<NP>([&](auto P) {
@@ -88,7 +105,8 @@ __device__ __forceinline__ Macro<R> macro(const Pairs<R>& p) {
   });
   Macro<R> m;
   m.rho = rho;
-  const R inv = R(1) / rho;
+  const R inv = recip<RCP>(rho);
+  m.inv = inv;
   m.ux = jx * inv;
   m.uy = jy * inv;
   m.uz = jz * inv;
@@ -96,9 +114,9 @@ __device__ __forceinline__ Macro<R> macro(const Pairs<R>& p) {
   return m;
 }
 
-template <typename R>
+template <bool RCP = false, typename R>
 __device__ __forceinline__ Macro<R> macro(const R* f) {
-  return macro(to_pairs(f));
+  return macro<RCP>(to_pairs(f));
 }
 
 // e_i . u for a compile-time direction
@@ -313,26 +331,46 @@ __device__ __forceinline__ void zou_he_out(R* f, R rho_out) {
   f[12] = f[13] - R(1.0 / 6.0) * rho_out * ux;
 }
 
-// Stability guard predicate (state.py:158-160; examples/10:71-74).
+// Largest u.u whose correctly rounded square root is <= 0.4 in R: rounded
+// sqrt is monotone, so sqrt(usq) > R(0.4) holds exactly when
+// usq > guard_usq<R>() (also for inf and NaN), and the guard needs no sqrt.
+template <typename R>
+__host__ __device__ constexpr R guard_usq();
+template <>
+__host__ __device__ constexpr float guard_usq<float>() { return 0x1.47ae16p-3f; }
+template <>
+__host__ __device__ constexpr double guard_usq<double>() { return 0x1.47ae147ae147cp-3; }
+
+// Stability guard predicate (state.py:158-160; examples/10:71-74):
+// rho < 0.3, rho > 2, |u| > 0.4 or rho NaN.
 template <typename R>
 __device__ __forceinline__ bool guard_bad(const Macro<R>& m) {
-  const R speed = sqrt(m.usq);
-  return (m.rho < R(0.3)) | (m.rho > R(2.0)) | (speed > R(0.4)) | (m.rho != m.rho);
+  return (m.rho < R(0.3)) | (m.rho > R(2.0)) | (m.usq > guard_usq<R>()) | (m.rho != m.rho);
 }
 
-template <typename R>
+template <bool RCP = false, typename R>
 __device__ __forceinline__ bool guard_bad(const R* f) {
-  return guard_bad(macro(f));
+  return guard_bad(macro<RCP>(f));
 }
 
-// f <- f[opp] (hard solid bounce, state.py:146-149)
+// f <- f[opp] (hard solid bounce, state.py:146-149).  For float the pairs
+// are exchanged in place with v_swap_b32: a swap the compiler renames
+// instead makes the cells that skip the (divergent) bounce copy 9 values at
+// the join.
 template <typename R>
 __device__ __forceinline__ void reverse_all(R* f) {
   sfor<NP>([&](auto P) {
     constexpr int a = PA[P], o = opp(PA[P]);
-    const R t = f[a];
-    f[a] = f[o];
-    f[o] = t;
+    if constexpr (sizeof(R) == 4) {
+      R x = f[a], y = f[o];
+      asm("v_swap_b32 %0, %1" : "+v"(x), "+v"(y));
+      f[a] = x;
+      f[o] = y;
+    } else {
+      const R t = f[a];
+      f[a] = f[o];
+      f[o] = t;
+    }
   });
 }
 
@@ -371,16 +409,16 @@ __device__ __forceinline__ void set_weights(R* f) {
 // the eddy viscosity and the collision (examples/10:71-78, state.py:133-145).
 // omega = 1/tau is passed in for the constant-tau case.  Returns the omega
 // used; sets `reset` when the guard fired.
-template <bool MRT, typename R>
+template <bool MRT, bool RCP = false, typename R>
 __device__ __forceinline__ R relax(R* f, R tau, R omega, bool pre_guard, R urx, R ury, R urz, bool smag, R cs,
                                    bool& reset) {
   Pairs<R> p = to_pairs(f);
-  Macro<R> m = macro(p);
+  Macro<R> m = macro<RCP>(p);
   reset = pre_guard && guard_bad(m);
   if (reset) {
     set_feq_unit(f, urx, ury, urz);
     p = to_pairs(f);
-    m = macro(p);
+    m = macro<RCP>(p);
   }
   if (smag) omega = R(1) / smag_tau(p, m, tau, cs);
   if constexpr (MRT) collide_mrt(f, omega, p, m);

@@ -5,6 +5,7 @@ import itertools
 import json
 import os
 
+import numpy as np
 import pytest
 import torch
 
@@ -227,6 +228,30 @@ def test_half_storage_rounds_to_nearest_even(dev, storage, tdt):
     lat = Lattice(nx, ny, nz, tau=0.6, storage=storage, offset=[0.0] * 19, device=dev)
     back = cpu(lat.from_state(lat.to_state(f.to(dev))))
     assert torch.equal(back, f.to(tdt).to(torch.float32))
+
+
+def test_half_storage_deviation_rounded_once(dev):
+    """With non-zero offsets the binary16 state holds f - off_i rounded once
+    to nearest even (the exact difference, as numpy's float64 -> float16
+    conversion rounds it), and decoding returns h + off_i rounded once to
+    float32.  Deviations span the subnormal and normal half range."""
+    nx, ny, nz = 64, 4, 4
+    n = nx * ny * nz
+    g = torch.Generator().manual_seed(11)
+    off = (torch.rand(19, generator=g, dtype=torch.float64) * 0.4 + 0.01).to(torch.float32)
+    dev_ = torch.randn(19, n, generator=g, dtype=torch.float64) * torch.exp2(
+        torch.randint(-26, -1, (19, n), generator=g).to(torch.float64))
+    f = (off.to(torch.float64)[:, None] + dev_).to(torch.float32)
+    lat = Lattice(nx, ny, nz, tau=0.6, storage="f16", offset=off.tolist(), device=dev)
+    fs = f.reshape(19, nx, ny, nz)
+    buf = lat.to_state(fs.to(dev))
+    raw = cpu(buf.permute(0, 3, 2, 1)).reshape(19, n).numpy()
+    exact = f.to(torch.float64).numpy() - off.to(torch.float64).numpy()[:, None]
+    want = exact.astype(np.float16)
+    assert np.array_equal(raw.view(np.uint16), want.view(np.uint16))
+    back = cpu(lat.from_state(buf)).reshape(19, n).numpy()
+    want_back = (want.astype(np.float64) + off.to(torch.float64).numpy()[:, None]).astype(np.float32)
+    assert np.array_equal(back, want_back)
 
 
 def test_default_state_layouts(dev):
