@@ -62,7 +62,7 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def build_lattice(storage="f32", nx=NX, ny=NY, nz=NZ, device="cuda"):
+def build_lattice(storage="f32", nx=NX, ny=NY, nz=NZ, device="cuda", rounding="stochastic"):
     import lbm_jax_autodiff_amd as lbm
     from lbm_jax_autodiff_amd.engine import Lattice
     u_in = 0.05
@@ -72,7 +72,7 @@ def build_lattice(storage="f32", nx=NX, ny=NY, nz=NZ, device="cuda"):
     solid, q = lbm.create_sphere(nx, ny, nz, (0.0, 0.0, 0.0), radius, device=device)
     lat = Lattice(nx, ny, nz, tau=tau, inlet_velocity=(u_in, 0.0, 0.0), collision="mrt", solid=solid, q=q,
                   guard="pre", guard_velocity=(u_in, 0.0, 0.0), periodic_yz=True, storage=storage,
-                  offset="inlet", device=device)
+                  offset="inlet", device=device, rounding=rounding)
     return lat, solid, q, tau
 
 
@@ -181,27 +181,32 @@ def small_grids(device, steps=2000):
 
 def dcd_dr_study(steps, device):
     """dCd/dr of the time-averaged Cd (last 30% of `steps`) for the 256x128x128
-    sphere with checkpoints every 100 steps, in float32, float16 (half
-    cotangents) and float64 storage: wall time and errors against float64."""
+    sphere with checkpoints every 100 steps, in float32, float16 and bfloat16
+    (half cotangents; stochastic rounding, and float16 also rounded to
+    nearest) and float64 storage: wall time and errors against float64."""
     from lbm_jax_autodiff_amd.protocols import sphere_loss_and_grad
     res = {}
     torch.cuda.empty_cache()
-    for st in ("f32", "f16", "f64"):
+    runs = (("f32", "f32", "stochastic"), ("f16", "f16", "stochastic"), ("f16_nearest", "f16", "nearest"),
+            ("bf16", "bf16", "stochastic"), ("f64", "f64", "stochastic"))
+    for key, st, rnd in runs:
         # a 200-step run of the same pipeline warms the allocator and the plans
         for n in (200, steps):
             r = sphere_loss_and_grad(NX, NY, NZ, n, (0.0, 0.0, 0.0), 0.5, storage=st, checkpoint=100,
-                                     avg_last=int(0.3 * n), guard="pre")
-        res[st] = r
+                                     avg_last=int(0.3 * n), guard="pre", rounding=rnd)
+        res[key] = r
         torch.cuda.empty_cache()
-        log(f"dCd/dr study {st}: {r['fwd_adj_s']:.2f} s, Cd={r['loss']:.7f}, dCd/dr={r['dloss_dr_world']:.7f}")
+        log(f"dCd/dr study {key}: {r['fwd_adj_s']:.2f} s, Cd={r['loss']:.7f}, dCd/dr={r['dloss_dr_world']:.7f}")
     ref = res["f64"]
-    out = {"steps": steps, "checkpoint": 100, "timing": "after a 200-step warm-up run of the same pipeline"}
-    for st in ("f32", "f16", "f64"):
+    out = {"steps": steps, "checkpoint": 100, "timing": "after a 200-step warm-up run of the same pipeline",
+           "rounding": "16-bit storage rounds stochastically, f16_nearest to nearest even"}
+    for st, _, _ in runs:
         r = res[st]
         out[st] = {"s": round(r["fwd_adj_s"], 3), "cd": r["loss"], "dcd_dr": r["dloss_dr_world"],
                    "cd_rel_err": abs(r["loss"] - ref["loss"]) / abs(ref["loss"]),
                    "dcd_dr_rel_err": abs(r["dloss_dr_world"] - ref["dloss_dr_world"]) / abs(ref["dloss_dr_world"])}
     out["f16_speedup_vs_f32"] = round(res["f32"]["fwd_adj_s"] / res["f16"]["fwd_adj_s"], 3)
+    out["f16_nearest_speedup_vs_f32"] = round(res["f32"]["fwd_adj_s"] / res["f16_nearest"]["fwd_adj_s"], 3)
     return out
 
 
@@ -324,13 +329,17 @@ def main():
     extras = {}
     if not args.no_extras:
         del lat
-        for st in ("f16", "bf16"):
-            lat_h, _, _, _ = build_lattice(st, device=dev)
+        # 16-bit storage with stochastic rounding (the default) and, for
+        # float16, rounded to nearest
+        for st, rnd, key in (("f16", "stochastic", "f16"), ("f16", "nearest", "f16_nearest"),
+                             ("bf16", "stochastic", "bf16")):
+            lat_h, _, _, _ = build_lattice(st, device=dev, rounding=rnd)
             lm, km, _ = time_forward(lat_h, f0, max(args.steps // 2, 10), args.warmup // 2)
-            extras[f"fwd_{st}_mlups"] = round(cells / (lm * 1e-3) / 1e6, 1)
-            extras[f"fwd_{st}_hbm_frac"] = round(76.0 * cells / (lm * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
-            extras[f"fwd_{st}_kernel_ms"] = round(km, 5)
-            log(f"{st} forward: {lm:.4f} ms/step, {extras[f'fwd_{st}_mlups']} MLUPS")
+            extras[f"fwd_{key}_mlups"] = round(cells / (lm * 1e-3) / 1e6, 1)
+            extras[f"fwd_{key}_hbm_frac"] = round(76.0 * cells / (lm * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+            extras[f"fwd_{key}_kernel_ms"] = round(lm, 5)
+            extras[f"fwd_{key}_single_launch_ms"] = round(km, 5)
+            log(f"{key} forward: {lm:.4f} ms/step, {extras[f'fwd_{key}_mlups']} MLUPS")
             del lat_h
         # forward + hand-written adjoint, store-all, dCd/dr of the time-averaged Cd
         adj_steps = args.adj_steps

@@ -253,7 +253,22 @@ typedef struct lbm_step_desc {
                                  row_stride >= 19 * pop_stride ([nz][ny][19][nx]
                                  for pop_stride = nx, row_stride = 19 nx: the
                                  19 populations of an x-row are adjacent) */
+  int32_t rounding;         /* 16-bit storage formats: LBM_ROUND_NEAREST
+                               (default) or LBM_ROUND_STOCHASTIC; ignored by
+                               the float32/float64 formats */
+  uint32_t seed;            /* stochastic rounding: seed of the dither */
+  int64_t step;             /* stochastic rounding: index of the first step
+                               of the call (lbm_step_chain's step k is
+                               step + k); the dither of a stored value is a
+                               hash of (seed, step, mode, cell, population),
+                               so recomputing a step reproduces it exactly */
 } lbm_step_desc;
+
+/* Rounding of the 16-bit storage formats.  NEAREST: round to nearest even.
+ * STOCHASTIC: the stored deviation is rounded up or down with probabilities
+ * that make its expectation the exact value (a deterministic hash supplies
+ * the uniform variate).  No counterpart in the reference (float32 only). */
+enum { LBM_ROUND_NEAREST = 0, LBM_ROUND_STOCHASTIC = 1 };
 
 /* Recommended pop_stride (>= N) for the grid of d: a padded stride that keeps
  * the 19 population streams of a cell out of the same HBM channels.  No
@@ -378,7 +393,9 @@ int lbm_adjoint_chain(const lbm_step_desc* d, const lbm_adjoint_desc* a, int n, 
                       void* lam_a, void* lam_b, const double* gF, lbm_stream_t stream);
 
 /* Convert between storage formats (e.g. float32 <-> half deviations) for a
- * full state; src and dst both use d->pop_stride and d->row_stride. */
+ * full state; src and dst both use d->pop_stride and d->row_stride.  Encoding
+ * into a 16-bit format follows d->rounding (stochastic: the dither of
+ * d->seed and d->step, drawn as by a step of its own). */
 int lbm_convert_state(const lbm_step_desc* d, int src_dtype, const void* src,
                       int dst_dtype, void* dst, lbm_stream_t stream);
 

@@ -19,6 +19,7 @@ LBM_COLL_BGK, LBM_COLL_MRT = 0, 1
 LBM_BOUNCE_NONE, LBM_BOUNCE_HARD, LBM_BOUNCE_SOFT = 0, 1, 2
 LBM_GUARD_NONE, LBM_GUARD_POST, LBM_GUARD_PRE = 0, 1, 2
 LBM_MODE_FULL, LBM_MODE_COLLIDE, LBM_MODE_STREAM = 0, 1, 2
+LBM_ROUND_NEAREST, LBM_ROUND_STOCHASTIC = 0, 1
 LBM_NOALLOC_PARTIALS = 1 << 20  # doubles of the no-allocation entry point's partials array
 
 _ERRORS = {1001: "LBM_E_ARG (bad size, pointer or flag)", 1002: "LBM_E_DTYPE (dtype not supported)",
@@ -43,7 +44,8 @@ class StepDesc(ctypes.Structure):
         ("flags", ctypes.c_void_p), ("porosity", ctypes.c_void_p),
         ("link_cell", ctypes.c_void_p), ("link_q", ctypes.c_void_p),
         ("cells_per_thread", ctypes.c_int32), ("pop_stride", ctypes.c_int64),
-        ("row_stride", ctypes.c_int64),
+        ("row_stride", ctypes.c_int64), ("rounding", ctypes.c_int32), ("seed", ctypes.c_uint32),
+        ("step", ctypes.c_int64),
     ]
 
 

@@ -142,14 +142,14 @@ class _Checkpoints:
         lat = self.lat
         if keep_every == 1:
             bufs = [p_start] + [lat.new_state() for _ in range(t1 - 1 - t0)]
-            lat.chain(bufs)
+            lat.chain(bufs, step=t0)
             return {t0 + k: b for k, b in enumerate(bufs)}
         out = {t0: p_start}
         sa, sb = lat.new_state(), lat.new_state()
         cur, t = p_start, t0
         while t < t1 - 1:
             end = min(t1 - 1, (t // keep_every + 1) * keep_every)
-            cur = lat.pingpong(cur, sa, sb, end - t)
+            cur = lat.pingpong(cur, sa, sb, end - t, step=t)
             if end % keep_every == 0:
                 out[end] = lat.clone_state(cur)
             t = end
@@ -196,7 +196,7 @@ class _RolloutFn(torch.autograd.Function):
                       q=q.detach() if q is not None else None, guard=cfg["guard"],
                       guard_velocity=cfg["guard_velocity"], periodic_yz=True, storage=cfg["storage"],
                       offset=cfg["offset"], device=f0.device, layout=cfg["layout"],
-                      cells_per_thread=cfg["cells_per_thread"])
+                      cells_per_thread=cfg["cells_per_thread"], rounding=cfg["rounding"], seed=cfg["seed"])
         ck = _Checkpoints(lat, cfg["checkpoint"], n_steps)
         s0 = lat.to_state(f0.detach())
         a = lat.new_state()
@@ -206,7 +206,7 @@ class _RolloutFn(torch.autograd.Function):
         if ck.store_all:
             # every P_t in a buffer of its own
             bufs = [a] + [lat.new_state() for _ in range(n_steps - 1)]
-            lat.chain(bufs, fp)
+            lat.chain(bufs, fp, step=0)
             for t, buf in enumerate(bufs):
                 ck.record(t, buf)
             cur = bufs[-1]
@@ -216,11 +216,11 @@ class _RolloutFn(torch.autograd.Function):
             while t < n_steps - 1:
                 ck.record(t, cur)
                 end = min(n_steps - 1, (t // ck.stride + 1) * ck.stride)
-                cur = lat.pingpong(cur, a, b, end - t, None if fp is None else fp[t:end])
+                cur = lat.pingpong(cur, a, b, end - t, None if fp is None else fp[t:end], step=t)
                 t = end
             ck.record(n_steps - 1, cur)
         s_final = lat.new_state()
-        lat.stream(cur, s_final, fpart[n_steps - 1] if lat.slots else None)
+        lat.stream(cur, s_final, fpart[n_steps - 1] if lat.slots else None, step=n_steps - 1)
         forces = lat.reduce_forces(fpart, n_steps) if lat.slots else torch.zeros(
             (n_steps, 3), dtype=torch.float64, device=lat.device)
         p_last = lat.clone_state(cur) if lat.storage in ("f16", "bf16") else cur
@@ -362,7 +362,8 @@ def rollout(f0: torch.Tensor, n_steps: int, *, tau, q: Optional[torch.Tensor] = 
             collision: str = "bgk", smagorinsky_cs: Optional[float] = None, guard: str = "none",
             guard_velocity=None, storage: str = "f32", offset="weights",
             checkpoint: Union[str, int, Sequence[int]] = "all", adjoint_storage: Optional[str] = None,
-            layout: str = "rows", cells_per_thread: int = 0, link_additions: str = "fold"):
+            layout: str = "rows", cells_per_thread: int = 0, link_additions: str = "fold",
+            rounding: str = "stochastic", seed: int = 0):
     """Differentiable fused rollout: (S_N, P_{N-1}, forces[N, 3] float64).
 
     Step order as in the reference's drivers: [pre guard] -> [Smagorinsky]
@@ -376,7 +377,9 @@ def rollout(f0: torch.Tensor, n_steps: int, *, tau, q: Optional[torch.Tensor] = 
     ``link_additions`` "fold" (default) lets each adjoint step add the
     previous step's Bouzidi and force cotangents as it loads them (one launch
     per step); "scatter" adds them with a separate launch per step from a
-    per-entry plan (the same sums in another order).
+    per-entry plan (the same sums in another order).  ``rounding`` and
+    ``seed`` set the 16-bit formats' rounding as for ``Lattice``; the
+    checkpoint recompute draws the same dither, so it reproduces the forward.
     """
     nx, ny, nz = F.dims_of_pop(f0)
     if isinstance(inlet_velocity, torch.Tensor):
@@ -391,7 +394,7 @@ def rollout(f0: torch.Tensor, n_steps: int, *, tau, q: Optional[torch.Tensor] = 
                collision=collision, smagorinsky_cs=smagorinsky_cs, solid=solid, guard=guard,
                guard_velocity=guard_velocity, storage=storage, offset=offset, checkpoint=checkpoint,
                adjoint_storage=adjoint_storage, layout=layout, cells_per_thread=int(cells_per_thread),
-               link_additions=link_additions)
+               link_additions=link_additions, rounding=rounding, seed=int(seed))
     if link_additions not in ("fold", "scatter"):
         raise ValueError("link_additions must be 'fold' or 'scatter'")
     if int(n_steps) < 1:

@@ -118,6 +118,8 @@ typedef double f64x2 __attribute__((ext_vector_type(2)));
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 
 // Raw buffer access (gfx9 buffer resource, untyped, 32-bit byte offsets): a
 // wave-uniform resource and scalar byte offset (soffset) plus a per-lane byte
@@ -150,6 +152,18 @@ __device__ __forceinline__ unsigned short f16_bits_of_diff(float v, float off) {
   uint32_t r;
   asm("v_fma_mixlo_f16 %0, %1, 1.0, -%2 op_sel_hi:[0,0,0]" : "=v"(r) : "v"(v), "s"(off));
   return static_cast<unsigned short>(r);
+}
+
+// 32-bit integer hash ("lowbias32", C. Wellons, 2018: xor-shift-multiply
+// rounds with the published constants).  Supplies the uniform variates of
+// stochastic rounding.
+__host__ __device__ constexpr uint32_t mix32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
 }
 
 struct StoreF32 {
@@ -252,6 +266,26 @@ struct StoreHalfDev {
   __device__ static R ld(const T* p, size_t k, R off) { return dec(__builtin_nontemporal_load(bits(p) + k), off); }
   __device__ static R ld_c(const T* p, size_t k, R off) { return dec(bits(p)[k], off); }
   __device__ static void st(T* p, size_t k, R v, R off) { __builtin_nontemporal_store(enc(v, off), bits(p) + k); }
+  // Stochastic rounding of the deviation d = v - off with gfx950's
+  // v_cvt_sr_f16_f32 / v_cvt_sr_bf16_f32: the 16-bit value above or below d,
+  // the upper one with probability (d - lo) / (hi - lo), decided by the
+  // random bits rbits (binary16 subnormals included).
+  __device__ static unsigned short enc_sr(float v, float off, uint32_t rbits) {
+    const float d = v - off;
+    if constexpr (BF) {
+      const bf16x2 h = __builtin_amdgcn_cvt_sr_bf16_f32(bf16x2{}, d, rbits, false);
+      return __builtin_bit_cast(u16x2, h)[0];
+    } else {
+      const f16x2 h = __builtin_amdgcn_cvt_sr_f16_f32(f16x2{}, d, rbits, false);
+      return __builtin_bit_cast(u16x2, h)[0];
+    }
+  }
+  __device__ static void st_sr(T* p, size_t k, R v, R off, uint32_t rbits) {
+    __builtin_nontemporal_store(enc_sr(v, off, rbits), bits(p) + k);
+  }
+  __device__ static void bst_sr(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, R v, R off, uint32_t rbits) {
+    __builtin_amdgcn_raw_buffer_store_b16(enc_sr(v, off, rbits), r, int(vo), int(so), kBufNT);
+  }
   __device__ static R bld(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, R off) {
     return dec(__builtin_amdgcn_raw_buffer_load_b16(r, int(vo), int(so), 0), off);
   }
@@ -283,6 +317,18 @@ struct StoreHalfDev {
     }
   }
 };
+template <class S>
+constexpr bool is_half_storage = sizeof(typename S::T) == 2;
+
+// The random word of population I of a cell, from the cell's hash h: h
+// rotated right by 5 I mod 32 (19 distinct rotations).
+template <int I>
+__device__ __forceinline__ uint32_t pop_bits(uint32_t h) {
+  constexpr int sh = (5 * I) & 31;
+  if constexpr (sh == 0) return h;
+  else return __builtin_amdgcn_alignbit(h, h, sh);
+}
+
 using StoreF16 = StoreHalfDev<false>;
 using StoreBF16 = StoreHalfDev<true>;
 

@@ -41,6 +41,8 @@ struct StepArgs {
   int coll, smag, bounce, guard, periodic, n_links, nlb, ndb;
   int row_waves;        // nx is a multiple of 64: each wave's cells share one (y, z) row
   uint32_t state_bytes;  // extent of a state buffer in bytes (below 2^32 when row_waves)
+  int sround;            // 16-bit storage rounds stochastically (dither hash seeded by `seed`)
+  uint32_t seed;         // this launch's dither seed (from the call's seed, step and mode)
   R tau, omega, cs;
   R uin[3];
   R rho_out;
@@ -163,11 +165,27 @@ __device__ __forceinline__ void collide_cell(const StepArgs<R>& a, R* s, uint32_
   }
 }
 
-// s -> the 19 populations at cell offset o
+// Hash of cell c for this launch's stochastic rounding.
+template <typename R>
+__device__ __forceinline__ uint32_t cell_hash(const StepArgs<R>& a, uint32_t c) {
+  return mix32(c ^ a.seed);
+}
+
+// s -> the 19 populations of cell c at cell offset o
 template <class S>
 __device__ __forceinline__ void store_cell(const StepArgs<typename S::R>& a, typename S::T* __restrict__ dst,
-                                           uint32_t o, const typename S::R* s) {
+                                           uint32_t o, const typename S::R* s, uint32_t c) {
   const size_t N = a.P;
+  if constexpr (is_half_storage<S>) {
+    if (a.sround) {
+      const uint32_t h = cell_hash(a, c);
+      sfor<Q>([&](auto I) {
+        constexpr int i = I;
+        S::st_sr(dst, i * N + o, s[i], a.off[i], pop_bits<i>(h));
+      });
+      return;
+    }
+  }
   sfor<Q>([&](auto I) {
     constexpr int i = I;
     S::st(dst, i * N + o, s[i], a.off[i]);
@@ -267,6 +285,20 @@ __device__ __forceinline__ void dense_vec(const StepArgs<typename S::R>& a, cons
   }
   // link cells are written by the link workgroups (dense MODE_COLLIDE writes all)
   const bool has_link = (MODE != LBM_MODE_COLLIDE) && (fl & 0x02020202u);
+  if constexpr (is_half_storage<S>) {
+    if (a.sround) {
+#pragma unroll
+      for (int j = 0; j < V; ++j) {
+        if (MODE != LBM_MODE_COLLIDE && ((fl >> (8 * j)) & 2u)) continue;
+        const uint32_t h = cell_hash(a, c0 + uint32_t(j));
+        sfor<Q>([&](auto I) {
+          constexpr int i = I;
+          S::st_sr(dst, i * N + o0 + j, s[j][i], a.off[i], pop_bits<i>(h));
+        });
+      }
+      return;
+    }
+  }
   sfor<Q>([&](auto I) {
     constexpr int i = I;
     R o[V];
@@ -337,6 +369,16 @@ __device__ __forceinline__ void dense_row(const StepArgs<typename S::R>& a, cons
   if (MODE == LBM_MODE_FULL) collide_cell<rcp_of<S>, COLL, SOFT>(a, s, c, fl);
   if (fl & 2u) return;  // owned by a link workgroup
   const __amdgpu_buffer_rsrc_t rout = buffer_rsrc(dst, a.state_bytes);
+  if constexpr (is_half_storage<S>) {
+    if (a.sround) {
+      const uint32_t h = cell_hash(a, c);
+      sfor<Q>([&](auto I) {
+        constexpr int i = I;
+        S::bst_sr(rout, bx0, (uint32_t(i) * a.P + own) * B, s[i], a.off[i], pop_bits<i>(h));
+      });
+      return;
+    }
+  }
   sfor<Q>([&](auto I) {
     constexpr int i = I;
     S::bst(rout, bx0, (uint32_t(i) * a.P + own) * B, s[i], a.off[i]);
@@ -507,7 +549,7 @@ __device__ __forceinline__ void link_block(const StepArgs<typename S::R>& a, con
   }
   if (live) {
     if (MODE == LBM_MODE_FULL) collide_cell<rcp_of<S>, COLL>(a, s, c, fl);
-    store_cell<S>(a, dst, o, s);
+    store_cell<S>(a, dst, o, s, c);
   }
   if (fpart && lane < 3) {
     double acc = 0.0;
@@ -571,7 +613,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(step_waves_p
     if (MODE == LBM_MODE_FULL) collide_cell<rcp_of<S>, COLL>(a, s, c, fl);
     if (fl & 2u) return;  // owned by a link workgroup
   }
-  store_cell<S>(a, dst, o, s);
+  store_cell<S>(a, dst, o, s, c);
 }
 
 __global__ void k_force_reduce(const double* __restrict__ part, int T, int slots, double* __restrict__ force) {
@@ -602,6 +644,14 @@ __global__ void k_convert(const typename SI::T* __restrict__ in, typename SO::T*
   using RO = typename SO::R;
   const float off = a.off[i];
   const RI v = SI::ld_c(in, e, RI(off));
+  if constexpr (is_half_storage<SO>) {
+    if (a.sround) {
+      const uint32_t h = cell_hash(a, c);
+      const uint32_t sh = uint32_t(5 * i) & 31u;
+      SO::st_sr(out, e, RO(v), RO(off), sh ? __builtin_amdgcn_alignbit(h, h, sh) : h);
+      return;
+    }
+  }
   SO::st(out, e, RO(v), RO(off));
 }
 
@@ -647,7 +697,17 @@ int make_args(const lbm_step_desc* d, int mode, StepArgs<R>& a) {
   a.ndb = int((n + kStepBlock - 1) / kStepBlock);
   a.row_waves = 0;
   a.state_bytes = 0;
+  a.sround = 0;
+  a.seed = 0;
+  if (d->rounding != LBM_ROUND_NEAREST && d->rounding != LBM_ROUND_STOCHASTIC) return LBM_E_ARG;
   return LBM_OK;
+}
+
+// Dither seed of one launch: a hash of the call's seed, the step index and
+// the mode, so a recomputed step draws the same variates.
+inline uint32_t launch_seed(uint32_t seed, int64_t step, int mode) {
+  const uint64_t t = uint64_t(step);
+  return mix32(seed ^ mix32(uint32_t(t) * 4u + uint32_t(mode)) ^ mix32(uint32_t(t >> 32) + 0x9e3779b9u));
 }
 
 // Cells per thread of the dense sweep: 2 or 4 when requested and nx allows,
@@ -690,11 +750,16 @@ int launch_step_cfg(const StepArgs<typename S::R>& a, int V, int bs, const typen
 }
 
 template <class S>
-int launch_step(const lbm_step_desc* d, int mode, const void* src, void* dst, double* fpart, hipStream_t s) {
+int launch_step(const lbm_step_desc* d, int mode, const void* src, void* dst, double* fpart, hipStream_t s,
+                int64_t step) {
   using R = typename S::R;
   StepArgs<R> a;
   const int rc = make_args(d, mode, a);
   if (rc) return rc;
+  if (is_half_storage<S> && d->rounding == LBM_ROUND_STOCHASTIC) {
+    a.sround = 1;
+    a.seed = launch_seed(d->seed, step, mode);
+  }
   if (!src || !dst || src == dst) return LBM_E_ARG;
   const int V = choose_vec(d, a.P, a.RS);
   {
@@ -781,17 +846,26 @@ int lbm_step_force_slots(const lbm_step_desc* d) {
   return (d->n_links + kLinkCellsPerBlock - 1) / kLinkCellsPerBlock;
 }
 
+}  // extern "C"
+
+// One launch of step `step` (the dither index of stochastic rounding).
+static int step_at(const lbm_step_desc* d, int mode, const void* src, void* dst, double* force_partial,
+                   hipStream_t s, int64_t step) {
+  switch (d->storage) {
+    case LBM_F32: return launch_step<StoreF32>(d, mode, src, dst, force_partial, s, step);
+    case LBM_F64: return launch_step<StoreF64>(d, mode, src, dst, force_partial, s, step);
+    case LBM_F16DEV: return launch_step<StoreF16>(d, mode, src, dst, force_partial, s, step);
+    case LBM_BF16DEV: return launch_step<StoreBF16>(d, mode, src, dst, force_partial, s, step);
+  }
+  return LBM_E_DTYPE;
+}
+
+extern "C" {
+
 int lbm_step(const lbm_step_desc* d, int mode, const void* src, void* dst, double* force_partial,
              lbm_stream_t stream) {
   if (!d) return LBM_E_ARG;
-  hipStream_t s = (hipStream_t)stream;
-  switch (d->storage) {
-    case LBM_F32: return launch_step<StoreF32>(d, mode, src, dst, force_partial, s);
-    case LBM_F64: return launch_step<StoreF64>(d, mode, src, dst, force_partial, s);
-    case LBM_F16DEV: return launch_step<StoreF16>(d, mode, src, dst, force_partial, s);
-    case LBM_BF16DEV: return launch_step<StoreBF16>(d, mode, src, dst, force_partial, s);
-  }
-  return LBM_E_DTYPE;
+  return step_at(d, mode, src, dst, force_partial, (hipStream_t)stream, d->step);
 }
 
 int lbm_step_chain(const lbm_step_desc* d, int n, void* const* states, double* force_partial,
@@ -799,7 +873,7 @@ int lbm_step_chain(const lbm_step_desc* d, int n, void* const* states, double* f
   if (!d || n < 0 || (n > 0 && !states) || partial_stride < 0) return LBM_E_ARG;
   for (int k = 0; k < n; ++k) {
     double* fp = force_partial ? force_partial + size_t(k) * size_t(partial_stride) : nullptr;
-    const int rc = lbm_step(d, LBM_MODE_FULL, states[k], states[k + 1], fp, stream);
+    const int rc = step_at(d, LBM_MODE_FULL, states[k], states[k + 1], fp, (hipStream_t)stream, d->step + k);
     if (rc) return rc;
   }
   return LBM_OK;
@@ -834,6 +908,11 @@ int lbm_convert_state(const lbm_step_desc* d, int src_dtype, const void* src, in
   const int rc = make_args(d, LBM_MODE_COLLIDE, a);
   if (rc) return rc;
   if (!src || !dst) return LBM_E_ARG;
+  if (d->rounding == LBM_ROUND_STOCHASTIC) {
+    // encoding into a 16-bit format draws its dither like a step (mode 3)
+    a.sround = 1;
+    a.seed = launch_seed(d->seed, d->step, 3);
+  }
   const dim3 grid((a.N + 255) / 256, Q);
   hipStream_t s = (hipStream_t)stream;
 #define LBM_CONV(SI, SO) hipLaunchKernelGGL((k_convert<SI, SO>), grid, dim3(256), 0, s, \

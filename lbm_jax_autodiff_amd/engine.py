@@ -26,6 +26,7 @@ _STORAGE = {
     "f16": (L.LBM_F16DEV, torch.float16, torch.float32),
     "bf16": (L.LBM_BF16DEV, torch.bfloat16, torch.float32),
 }
+_ROUNDING = {"nearest": L.LBM_ROUND_NEAREST, "stochastic": L.LBM_ROUND_STOCHASTIC}
 _GUARD = {"none": L.LBM_GUARD_NONE, "post": L.LBM_GUARD_POST, "pre": L.LBM_GUARD_PRE}
 _W = [1.0 / 3.0] + [1.0 / 18.0] * 6 + [1.0 / 36.0] * 12
 _E = [(0, 0, 0), (1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1), (1, 1, 0), (-1, 1, 0),
@@ -113,18 +114,28 @@ class Lattice:
     element distance between consecutive populations (rows: nx by default;
     apart: the library's padded recommendation by default, N for the dense
     (19, nz, ny, nx) array).
+    ``rounding`` of the 16-bit formats: "stochastic" (default) -- each
+    stored deviation is rounded up or down with probabilities that make its
+    expectation the exact value (gfx950's stochastic-rounding conversions,
+    fed a hash of (``seed``, time index, cell, population), so recomputing a
+    step reproduces its bits) -- or "nearest" (even).  Rounding to nearest
+    leaves a steady flow's stored values with a fixed bias (a change smaller
+    than half a unit in the last place never lands); stochastic rounding
+    makes those errors average out over the steps.
     """
 
     def __init__(self, nx, ny, nz, *, tau, inlet_velocity=(0.05, 0.0, 0.0), rho_out=1.0, collision="bgk",
                  smagorinsky_cs=None, solid=None, porosity=None, q=None, guard="none", guard_velocity=None,
                  periodic_yz=True, storage="f32", offset="weights", device=None, dqdr=None,
-                 cells_per_thread=0, pop_stride=None, layout="rows"):
+                 cells_per_thread=0, pop_stride=None, layout="rows", rounding="stochastic", seed=0):
         if storage not in _STORAGE:
             raise ValueError(f"storage must be one of {sorted(_STORAGE)}")
         if guard not in _GUARD:
             raise ValueError(f"guard must be one of {sorted(_GUARD)}")
         if collision not in ("bgk", "mrt"):
             raise ValueError("collision must be 'bgk' or 'mrt'")
+        if rounding not in _ROUNDING:
+            raise ValueError(f"rounding must be one of {sorted(_ROUNDING)}")
         self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
         self.N = self.nx * self.ny * self.nz
         self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
@@ -195,6 +206,10 @@ class Lattice:
         d.link_cell = L.ptr(self.links.cell)
         d.link_q = L.ptr(self.links.q)
         d.cells_per_thread = int(cells_per_thread)
+        d.rounding = _ROUNDING[rounding]
+        d.seed = int(seed) & 0xFFFFFFFF
+        d.step = 0
+        self.rounding = rounding
         lib = L.load()
         if layout == "rows":
             self.P = self.nx if pop_stride is None else int(pop_stride)
@@ -227,8 +242,10 @@ class Lattice:
         out.copy_(buf)
         return out
 
-    def to_state(self, f: torch.Tensor) -> torch.Tensor:
-        """(19, nx, ny, nz) float tensor -> state buffer in the storage format."""
+    def to_state(self, f: torch.Tensor, step: int = 0) -> torch.Tensor:
+        """(19, nx, ny, nz) float tensor -> state buffer in the storage format
+        (16-bit formats: rounded as ``rounding`` says, with the dither of time
+        index ``step``)."""
         F.require_cuda(f, "f")
         fs = f.permute(0, 3, 2, 1)
         if self.storage in ("f32", "f64"):
@@ -238,6 +255,7 @@ class Lattice:
         src = self.new_state(torch.float32)
         src.copy_(fs)
         out = self.new_state()
+        self.desc.step = int(step)
         L.call("lbm_convert_state", ctypes.byref(self.desc), L.LBM_F32, L.ptr(src), self.tag, L.ptr(out),
                L.stream_handle(self.device))
         return out
@@ -252,26 +270,31 @@ class Lattice:
         return F.pop_view(out)
 
     # ---- single launches -------------------------------------------------
-    def launch(self, mode: int, src: torch.Tensor, dst: torch.Tensor, fpart: Optional[torch.Tensor] = None):
+    # `step` is the time index t of the launch (P_{t+1} from P_t; the
+    # prologue and the epilogue use their own t): with stochastic rounding it
+    # selects the dither, so a recomputed step reproduces its stored values.
+    def launch(self, mode: int, src: torch.Tensor, dst: torch.Tensor, fpart: Optional[torch.Tensor] = None,
+               step: int = 0):
+        self.desc.step = int(step)
         L.call("lbm_step", ctypes.byref(self.desc), int(mode), L.ptr(src), L.ptr(dst), L.ptr(fpart),
                L.stream_handle(self.device))
 
-    def collide(self, s_buf, p_buf):
+    def collide(self, s_buf, p_buf, step: int = 0):
         """P_t = bounce(collide(S_t)) (rollout prologue)."""
-        self.launch(L.LBM_MODE_COLLIDE, s_buf, p_buf)
+        self.launch(L.LBM_MODE_COLLIDE, s_buf, p_buf, step=step)
 
-    def step(self, p_in, p_out, fpart=None):
+    def step(self, p_in, p_out, fpart=None, step: int = 0):
         """P_{t+1} from P_t in one launch; fpart receives this step's force partials."""
-        self.launch(L.LBM_MODE_FULL, p_in, p_out, fpart)
+        self.launch(L.LBM_MODE_FULL, p_in, p_out, fpart, step=step)
 
-    def stream(self, p_in, s_out, fpart=None):
+    def stream(self, p_in, s_out, fpart=None, step: int = 0):
         """S_{t+1} from P_t (rollout epilogue)."""
-        self.launch(L.LBM_MODE_STREAM, p_in, s_out, fpart)
+        self.launch(L.LBM_MODE_STREAM, p_in, s_out, fpart, step=step)
 
-    def chain(self, states, fpart: Optional[torch.Tensor] = None):
-        """len(states) - 1 full steps in one host call: states[k] -> states[k+1].
-        fpart, if given, is a contiguous (n, slots, 3) float64 block receiving
-        each step's force partials."""
+    def chain(self, states, fpart: Optional[torch.Tensor] = None, step: int = 0):
+        """len(states) - 1 full steps in one host call: states[k] -> states[k+1]
+        (time indices step, step + 1, ...).  fpart, if given, is a contiguous
+        (n, slots, 3) float64 block receiving each step's force partials."""
         n = len(states) - 1
         if n <= 0:
             return
@@ -279,10 +302,11 @@ class Lattice:
         if fpart is not None and self.slots > 0:
             assert fpart.is_contiguous() and fpart.shape[0] >= n
             fp, stride = fpart.data_ptr(), 3 * self.slots
+        self.desc.step = int(step)
         L.call("lbm_step_chain", ctypes.byref(self.desc), n, L.ptr_array(states), fp, stride,
                L.stream_handle(self.device))
 
-    def chain_forces_noalloc(self, states, force: torch.Tensor):
+    def chain_forces_noalloc(self, states, force: torch.Tensor, step: int = 0):
         """len(states) - 1 full steps and their forces (float64, (n, 3)) in one
         call that allocates no device memory: the force partials live in the
         library's own device array (lbm_step_chain_forces_noalloc), which
@@ -297,18 +321,20 @@ class Lattice:
         F.require_shape(force, "force", (n, 3), self.device)
         if force.dtype != torch.float64 or not force.is_contiguous():
             raise ValueError("force must be a contiguous float64 tensor")
+        self.desc.step = int(step)
         L.call("lbm_step_chain_forces_noalloc", ctypes.byref(self.desc), n, L.ptr_array(states), L.ptr(force),
                L.stream_handle(self.device))
 
-    def pingpong(self, cur, spare_a, spare_b, n, fpart=None):
-        """n full steps from cur through two spare buffers (never writing cur
-        unless it is one of them); returns the buffer holding the result."""
+    def pingpong(self, cur, spare_a, spare_b, n, fpart=None, step: int = 0):
+        """n full steps from cur (time index `step`) through two spare buffers
+        (never writing cur unless it is one of them); returns the buffer
+        holding the result."""
         if n <= 0:
             return cur
         first = spare_a if cur is not spare_a else spare_b
         second = spare_b if first is spare_a else spare_a
         seq = [cur] + [first if k % 2 == 0 else second for k in range(n)]
-        self.chain(seq, fpart)
+        self.chain(seq, fpart, step=step)
         return seq[-1]
 
     def reduce_forces(self, fpart: torch.Tensor, T: int) -> torch.Tensor:
@@ -345,12 +371,12 @@ class Lattice:
             end = n_steps - 1
             if keep is not None and keep_every:
                 end = min(end, (t // keep_every + 1) * keep_every)
-            cur = self.pingpong(cur, a, b, end - t, None if fpart is None else fpart[t:end])
+            cur = self.pingpong(cur, a, b, end - t, None if fpart is None else fpart[t:end], step=t)
             t = end
         if keep is not None and keep_every and (n_steps - 1) % keep_every == 0:
             keep.append((n_steps - 1, self.clone_state(cur)))
         s_final = s0  # reuse the prologue input buffer for S_N
-        self.stream(cur, s_final, None if fpart is None else fpart[n_steps - 1])
+        self.stream(cur, s_final, None if fpart is None else fpart[n_steps - 1], step=n_steps - 1)
         F_hist = None
         if forces:
             F_hist = self.reduce_forces(fpart, n_steps) if fpart is not None else torch.zeros(

@@ -146,7 +146,8 @@ def sphere_cd(nx, ny, nz, n_steps, center_world, radius=0.5, *, init_u=0.0, stor
 def sphere_loss_and_grad(nx, ny, nz, n_steps, center_world, radius=0.5, *, avg_last=600, q_mode="bouzidi",
                          porosity_mode="hard", sharpness=20.0, collision="mrt", re=100.0, u_in=0.05,
                          storage="f32", checkpoint="all", smagorinsky_cs=None, guard="none", init_u=0.05,
-                         offset="inlet", geometry_dtype=None, r_ref=0.5, adjoint_storage=None):
+                         offset="inlet", geometry_dtype=None, r_ref=0.5, adjoint_storage=None,
+                         rounding="stochastic"):
     """loss = mean Cd over the last `avg_last` steps and dloss/dr (world units)
     through q(r) (Bouzidi) and porosity(r) (soft), examples/11:56-167.  The
     initial state is f_eq(1, (init_u, 0, 0)) (examples/11:147-149 use u_inlet).  The
@@ -184,7 +185,7 @@ def sphere_loss_and_grad(nx, ny, nz, n_steps, center_world, radius=0.5, *, avg_l
                            inlet_velocity=(u_in, 0.0, 0.0),
                            collision=collision, smagorinsky_cs=smagorinsky_cs, guard=guard,
                            guard_velocity=(u_in, 0.0, 0.0), storage=storage, offset=offset, checkpoint=checkpoint,
-                           adjoint_storage=adjoint_storage)
+                           adjoint_storage=adjoint_storage, rounding=rounding)
     cd = drag_series(forces, u_in, area)
     loss = cd[-avg_last:].mean()
     torch.cuda.synchronize()
@@ -300,6 +301,8 @@ def main(argv=None):
     ap.add_argument("--offset", default="inlet",
                     help="c4/c5 half storage: 'inlet' (f_eq(1, U)), 'weights' (w_i) or a number k (f_eq(1, k U))")
     ap.add_argument("--fd-eps", type=float, default=0.0, help="ex12: central-difference step (the script: 1e-3)")
+    ap.add_argument("--rounding", default="stochastic", choices=["nearest", "stochastic"],
+                    help="c4/c5 16-bit storage rounding (default stochastic)")
     args = ap.parse_args(argv)
     offset = args.offset
     if offset not in ("inlet", "weights"):
@@ -390,8 +393,10 @@ def main(argv=None):
             ck = parts[0] if len(parts) == 1 else tuple(parts)
         res = sphere_loss_and_grad(nx, ny, nz, steps, (0.0, 0.0, 0.0), 0.5, storage=args.storage,
                                    checkpoint=ck, avg_last=int(0.3 * steps), guard="pre",
-                                   adjoint_storage=args.adjoint_storage, offset=offset)
+                                   adjoint_storage=args.adjoint_storage, offset=offset,
+                                   rounding=args.rounding)
         res["config"] = "c4"
+        res["rounding"] = args.rounding if args.storage in ("f16", "bf16") else None
         _emit(res)
     elif args.config == "c5":
         nx, ny, nz = 384, 192, 192
@@ -403,13 +408,15 @@ def main(argv=None):
         ck = {"f32": (400, 20), "f64": (1200, 35)}.get(args.storage, (200, 14))
         if args.checkpoint:
             ck = tuple(int(v) for v in args.checkpoint.split(","))
+        rounding = args.rounding
         res = sphere_loss_and_grad(nx, ny, nz, steps, (0.0, 0.0, 0.0), 0.5, storage=args.storage,
                                    checkpoint=ck, avg_last=int(0.3 * steps), guard="pre",
                                    porosity_mode="soft" if soft else "hard", sharpness=10.0,
                                    smagorinsky_cs=0.1 if soft else None, adjoint_storage=args.adjoint_storage,
-                                   offset=offset)
+                                   offset=offset, rounding=rounding)
         res["config"] = "c5-" + args.mode
         res["offset"] = args.offset
+        res["rounding"] = rounding if args.storage in ("f16", "bf16") else None
         _emit(res)
 
 

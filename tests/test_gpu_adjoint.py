@@ -235,6 +235,38 @@ def test_checkpointed_gradient_equals_store_all(dev, policy):
     assert torch.equal(out[0], out[1])
 
 
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_stochastic_rounding_recompute_reproduces_forward(dev, storage):
+    """With stochastic rounding the dither of every stored value is a hash of
+    (seed, time index, cell, population): one-level and two-level
+    checkpointed rollouts recompute the forward's states bit for bit, so
+    forces, states and gradients equal the store-all run's; another seed
+    gives other states; the result stays close to the float32 run."""
+    nx, ny, nz = 24, 12, 12
+    por, solid, q = lbm.smooth_sphere_geometry(nx, ny, nz, (-1.0, 0.1, 0.0), 0.6, 10.0, device=dev)
+    u = torch.zeros((3, nx, ny, nz), device=dev)
+    u[0] = 0.05
+    f0 = lbm.equilibrium(torch.ones((nx, ny, nz), device=dev), u)
+    out = []
+    for ck, seed, st in (("all", 3, storage), (5, 3, storage), ((7, 3), 3, storage), ("all", 4, storage),
+                         ("all", 3, "f32")):
+        qq = q.clone().requires_grad_(True)
+        pp = por.clone().requires_grad_(True)
+        S, P, F = A.rollout(f0, 23, tau=0.56, q=qq, porosity=pp, collision="mrt", guard="pre", storage=st,
+                            offset="inlet", checkpoint=ck, rounding="stochastic", seed=seed,
+                            smagorinsky_cs=0.1)
+        loss = F[:, 0].sum() + (S.double() ** 2).sum() * 1e-3
+        out.append([cpu(t).double() for t in (S, F) + torch.autograd.grad(loss, (qq, pp))])
+    for a, b in zip(out[0], out[1]):
+        assert torch.equal(a, b)
+    for a, b in zip(out[0], out[2]):
+        assert torch.equal(a, b)
+    assert not torch.equal(out[0][0], out[3][0])
+    tol = 1e-2 if storage == "f16" else 5e-2
+    for a, b in zip(out[0], out[4]):
+        assert float((a - b).abs().max()) <= tol * max(1.0, float(b.abs().max()))
+
+
 @pytest.mark.parametrize("storage,tol", [("f64", 1e-13), ("f32", 2e-6), ("f16", 2e-3)])
 @pytest.mark.parametrize("checkpoint", ["all", (7, 3)])
 def test_folded_link_additions_match_scatter(dev, storage, tol, checkpoint):

@@ -225,7 +225,7 @@ def test_half_storage_rounds_to_nearest_even(dev, storage, tdt):
         up = torch.nextafter(h, torch.full_like(h, float("inf")))
         f[: n // 4] = ((h.to(torch.float64) + up.to(torch.float64)) / 2).to(torch.float32)
     f = f.reshape(19, nx, ny, nz)
-    lat = Lattice(nx, ny, nz, tau=0.6, storage=storage, offset=[0.0] * 19, device=dev)
+    lat = Lattice(nx, ny, nz, tau=0.6, storage=storage, offset=[0.0] * 19, device=dev, rounding="nearest")
     back = cpu(lat.from_state(lat.to_state(f.to(dev))))
     assert torch.equal(back, f.to(tdt).to(torch.float32))
 
@@ -242,7 +242,7 @@ def test_half_storage_deviation_rounded_once(dev):
     dev_ = torch.randn(19, n, generator=g, dtype=torch.float64) * torch.exp2(
         torch.randint(-26, -1, (19, n), generator=g).to(torch.float64))
     f = (off.to(torch.float64)[:, None] + dev_).to(torch.float32)
-    lat = Lattice(nx, ny, nz, tau=0.6, storage="f16", offset=off.tolist(), device=dev)
+    lat = Lattice(nx, ny, nz, tau=0.6, storage="f16", offset=off.tolist(), device=dev, rounding="nearest")
     fs = f.reshape(19, nx, ny, nz)
     buf = lat.to_state(fs.to(dev))
     raw = cpu(buf.permute(0, 3, 2, 1)).reshape(19, n).numpy()
@@ -252,6 +252,60 @@ def test_half_storage_deviation_rounded_once(dev):
     back = cpu(lat.from_state(buf)).reshape(19, n).numpy()
     want_back = (want.astype(np.float64) + off.to(torch.float64).numpy()[:, None]).astype(np.float32)
     assert np.array_equal(back, want_back)
+
+
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_stochastic_rounding_is_unbiased(dev, storage):
+    """With rounding="stochastic" each stored deviation is one of the two
+    16-bit neighbours of the exact deviation, the upper one with probability
+    (d - lo) / (hi - lo): over 256 time indices the mean stored value matches
+    the exact value (no bias over all values to a small fraction of an ulp),
+    each value's frequency agrees with its probability, the dither is a
+    function of (seed, step) only, and a different seed draws differently.
+    Deviations span the binary16 subnormals (float16) and lie well inside
+    their binades, so the float32 input holds each one to far below a 16-bit
+    unit in the last place."""
+    nx, ny, nz = 64, 4, 4
+    n = nx * ny * nz
+    g = torch.Generator().manual_seed(5)
+    off = (torch.rand(19, generator=g, dtype=torch.float64) * 0.3 + 0.02).to(torch.float32)
+    lowest = -22 if storage == "f16" else -12
+    mag = torch.exp2(torch.randint(lowest, -3, (19, n), generator=g).to(torch.float64))
+    d = mag * (1.05 + 0.9 * torch.rand(19, n, generator=g, dtype=torch.float64))  # inside [2^e, 2^(e+1))
+    d = d * torch.where(torch.rand(19, n, generator=g) < 0.5, -1.0, 1.0).to(torch.float64)
+    f = (off.to(torch.float64)[:, None] + d).to(torch.float32)
+    exact = f.to(torch.float64).numpy() - off.to(torch.float64).numpy()[:, None]
+    lat = Lattice(nx, ny, nz, tau=0.6, storage=storage, offset=off.tolist(), device=dev, rounding="stochastic",
+                  seed=123)
+    fs = f.reshape(19, nx, ny, nz).to(dev)
+
+    def stored(step, latt=lat):
+        buf = latt.to_state(fs, step=step)
+        raw = cpu(buf.permute(0, 3, 2, 1)).reshape(19, n)
+        return raw.to(torch.float64).numpy()
+
+    K = 256
+    draws = np.stack([stored(k) for k in range(K)])
+    # neighbours of the exact deviation in the 16-bit format
+    tdt = torch.float16 if storage == "f16" else torch.bfloat16
+    near = torch.tensor(exact).to(tdt)
+    towards = torch.where(torch.tensor(exact) > near.to(torch.float64), torch.full_like(near, float("inf")),
+                          torch.full_like(near, float("-inf")))
+    other = torch.nextafter(near, towards)
+    lo = torch.minimum(near, other).to(torch.float64).numpy()
+    hi = torch.maximum(near, other).to(torch.float64).numpy()
+    assert np.all((draws == lo[None]) | (draws == hi[None]))
+    p = (exact - lo) / (hi - lo)
+    freq = (draws == hi[None]).mean(0)
+    # per value: binomial spread of K draws
+    assert np.abs(freq - p).max() < 6.0 * np.sqrt(0.25 / K)
+    # over all values: the mean error in ulps has no bias
+    bias = ((draws.mean(0) - exact) / (hi - lo)).mean()
+    assert abs(bias) < 6.0 * np.sqrt(0.25 / K / exact.size), bias
+    assert np.array_equal(stored(7), draws[7])
+    lat2 = Lattice(nx, ny, nz, tau=0.6, storage=storage, offset=off.tolist(), device=dev, rounding="stochastic",
+                   seed=124)
+    assert not np.array_equal(stored(7, lat2), draws[7])
 
 
 def test_default_state_layouts(dev):
