@@ -341,6 +341,22 @@ def main():
             extras[f"fwd_{key}_single_launch_ms"] = round(km, 5)
             log(f"{key} forward: {lm:.4f} ms/step, {extras[f'fwd_{key}_mlups']} MLUPS")
             del lat_h
+        # the largest single-GPU sphere (384x192x192, D = 48): forward rate
+        c5 = {}
+        f0_big = None
+        for st, bpl in (("f32", 152.0), ("f16", 76.0)):
+            lat_c5, _, _, _ = build_lattice(st, 384, 192, 192, device=dev)
+            if f0_big is None:
+                f0_big = initial_state(lat_c5, dev)
+            lm, _, _ = time_forward(lat_c5, f0_big, 100, 10)
+            n5 = lat_c5.nx * lat_c5.ny * lat_c5.nz
+            c5[st] = {"us_per_step": round(lm * 1e3, 1), "mlups": round(n5 / (lm * 1e-3) / 1e6, 1),
+                      "hbm_frac": round(bpl * n5 / (lm * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+            log(f"384x192x192 {st} forward: {lm * 1e3:.1f} us/step, {c5[st]['mlups']} MLUPS")
+            del lat_c5
+        del f0_big
+        torch.cuda.empty_cache()
+        extras["c5_fwd_384x192x192"] = c5
         # forward + hand-written adjoint, store-all, dCd/dr of the time-averaged Cd
         adj_steps = args.adj_steps
         for st, bpl in (("f32", 380.0), ("f16", 190.0)):

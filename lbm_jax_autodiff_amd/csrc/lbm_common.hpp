@@ -376,6 +376,38 @@ inline bool state_layout(const lbm_step_desc* d, long long& pstride, long long& 
 // 9.7 -> 8.6 us; 256 x 128 x 128 unchanged within 1%.
 constexpr int kLinkCellsPerBlock = 64;
 
+// Sum of a float64 over the 64 lanes of a wave (every lane active), in every
+// lane: DPP exchanges inside each row of 16 lanes (quad_perm [1,0,3,2] and
+// [2,3,0,1], row_half_mirror, row_mirror) leave each lane with its row's
+// sum, then the four row sums are read from lanes 0, 16, 32 and 48.  Only
+// VALU operations: the shuffle form (ds_bpermute through LDS) puts six LDS
+// round trips on the dependent path.
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+  auto step = [](double x, auto ctrl) {
+    const u32x2 b = __builtin_bit_cast(u32x2, x);
+    u32x2 o;
+    o[0] = __builtin_amdgcn_update_dpp(0u, b[0], decltype(ctrl)::value, 0xf, 0xf, false);
+    o[1] = __builtin_amdgcn_update_dpp(0u, b[1], decltype(ctrl)::value, 0xf, 0xf, false);
+    return x + __builtin_bit_cast(double, o);
+  };
+  v = step(v, std::integral_constant<int, 0xB1>{});   // quad_perm [1,0,3,2]
+  v = step(v, std::integral_constant<int, 0x4E>{});   // quad_perm [2,3,0,1]
+  v = step(v, std::integral_constant<int, 0x141>{});  // row_half_mirror
+  v = step(v, std::integral_constant<int, 0x140>{});  // row_mirror
+  auto lane_value = [&](auto l) {
+    const u32x2 b = __builtin_bit_cast(u32x2, v);
+    u32x2 o;
+    o[0] = __builtin_amdgcn_readlane(b[0], decltype(l)::value);
+    o[1] = __builtin_amdgcn_readlane(b[1], decltype(l)::value);
+    return __builtin_bit_cast(double, o);
+  };
+  const double r0 = lane_value(std::integral_constant<int, 0>{});
+  const double r1 = lane_value(std::integral_constant<int, 16>{});
+  const double r2 = lane_value(std::integral_constant<int, 32>{});
+  const double r3 = lane_value(std::integral_constant<int, 48>{});
+  return (r0 + r1) + (r2 + r3);
+}
+
 template <typename R>
 __device__ __forceinline__ R wave_sum(R v) {
 #pragma unroll

@@ -483,6 +483,7 @@ __device__ __forceinline__ void link_block(const StepArgs<typename S::R>& a, con
   __shared__ R sl[Q][kLinkCellsPerBlock];
   __shared__ uint32_t lm[NW][kLinkCellsPerBlock];
   __shared__ double red[NW][3];
+  __shared__ double fw[3][kLinkCellsPerBlock], gw[3][kLinkCellsPerBlock];
   const int lane = int(threadIdx.x & 63u);
   const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
   const int b = int(bid) * kLinkCellsPerBlock + lane;
@@ -502,16 +503,29 @@ __device__ __forceinline__ void link_block(const StepArgs<typename S::R>& a, con
     });
   }
   lm[w][lane] = mask;
-  if (fpart) {
-    const double fx = wave_sum(f[0]), fy = wave_sum(f[1]), fz = wave_sum(f[2]);
-    if (lane == 0) {
-      red[w][0] = fx;
-      red[w][1] = fy;
-      red[w][2] = fz;
-    }
-  }
   __syncthreads();
-  if (w != 0) return;
+  // The float64 force sums stay off the collision's path: waves 1.. reduce
+  // their partials while wave 0 collides, and wave 0 parks its per-lane
+  // partials in LDS and reduces them after its stores are issued.  With the
+  // DPP reduction (wave_sum_dpp) this took 64 x 32 x 32 from 6.25 to
+  // 5.9 us per step (5.75 us without forces).
+  if (w != 0) {
+    if (fpart) {
+      const double fx = wave_sum_dpp(f[0]), fy = wave_sum_dpp(f[1]), fz = wave_sum_dpp(f[2]);
+      if (lane == 0) {
+        red[w][0] = fx;
+        red[w][1] = fy;
+        red[w][2] = fz;
+      }
+      __syncthreads();
+    }
+    return;
+  }
+  if (fpart) {
+    fw[0][lane] = f[0];
+    fw[1][lane] = f[1];
+    fw[2][lane] = f[2];
+  }
   R s[Q];
   double g[3] = {0.0, 0.0, 0.0};
   const bool face = live && (x == 0 || x == a.nx - 1 || a.guard == LBM_GUARD_POST);
@@ -537,25 +551,40 @@ __device__ __forceinline__ void link_block(const StepArgs<typename S::R>& a, con
       }
     });
   }
-  // the pulled part of the x-face cells joins wave 0's partial before the
-  // collision, which needs every register the dense budget leaves
-  if (fpart && __any(face)) {
-    const double gx = wave_sum(g[0]), gy = wave_sum(g[1]), gz = wave_sum(g[2]);
-    if (lane == 0) {
-      red[0][0] += gx;
-      red[0][1] += gy;
-      red[0][2] += gz;
-    }
+  // the pulled part of the x-face cells waits in LDS too: the collision
+  // needs every register the dense budget leaves
+  const bool any_face = __any(face);
+  if (fpart && any_face) {
+    gw[0][lane] = g[0];
+    gw[1][lane] = g[1];
+    gw[2][lane] = g[2];
   }
   if (live) {
     if (MODE == LBM_MODE_FULL) collide_cell<rcp_of<S>, COLL>(a, s, c, fl);
     store_cell<S>(a, dst, o, s, c);
   }
-  if (fpart && lane < 3) {
-    double acc = 0.0;
+  if (fpart) {
+    const double fx = wave_sum_dpp(fw[0][lane]), fy = wave_sum_dpp(fw[1][lane]), fz = wave_sum_dpp(fw[2][lane]);
+    if (lane == 0) {
+      red[0][0] = fx;
+      red[0][1] = fy;
+      red[0][2] = fz;
+    }
+    if (any_face) {
+      const double gx = wave_sum_dpp(gw[0][lane]), gy = wave_sum_dpp(gw[1][lane]), gz = wave_sum_dpp(gw[2][lane]);
+      if (lane == 0) {
+        red[0][0] += gx;
+        red[0][1] += gy;
+        red[0][2] += gz;
+      }
+    }
+    __syncthreads();
+    if (lane < 3) {
+      double acc = 0.0;
 #pragma unroll
-    for (int k = 0; k < NW; ++k) acc += red[k][lane];
-    fpart[size_t(bid) * 3 + lane] = acc;
+      for (int k = 0; k < NW; ++k) acc += red[k][lane];
+      fpart[size_t(bid) * 3 + lane] = acc;
+    }
   }
 }
 
