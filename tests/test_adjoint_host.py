@@ -36,15 +36,18 @@ class _CountingLattice:
         self.steps += 1
         dst.copy_(src + 1)
 
-    def chain(self, states, fpart=None):
+    def chain(self, states, fpart=None, step=0):
+        # the time index handed down must be that of the state (the
+        # stochastic-rounding dither of a recomputed step depends on it)
         for k in range(len(states) - 1):
+            assert float(states[k]) == float(step + k)
             self.step(states[k], states[k + 1])
 
-    def pingpong(self, cur, spare_a, spare_b, n, fpart=None):
+    def pingpong(self, cur, spare_a, spare_b, n, fpart=None, step=0):
         first = spare_a if cur is not spare_a else spare_b
         second = spare_b if first is spare_a else spare_a
         seq = [cur] + [first if k % 2 == 0 else second for k in range(n)]
-        self.chain(seq)
+        self.chain(seq, step=step)
         return seq[-1]
 
 
@@ -101,7 +104,7 @@ def test_checkpoint_peak_states(policy, n):
     while t < n - 1:
         ck.record(t, cur)
         end = min(n - 1, (t // ck.stride + 1) * ck.stride)
-        cur = lat.pingpong(cur, a, b, end - t)
+        cur = lat.pingpong(cur, a, b, end - t, step=t)
         t = end
     ck.record(n - 1, cur)
     expect_t = n - 1
