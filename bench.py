@@ -272,11 +272,29 @@ def cpu_baseline(sample_steps=2):
     n1 = max(1, sample_steps // 8)
     dt1 = run(n1)
     torch.set_num_threads(cores)
+
+    # forward + reverse mode: torch autograd of the same steps, the gradient
+    # of the summed drag force with respect to q (the reference's jax.grad
+    # path through the Bouzidi links)
+    n_adj = max(1, sample_steps // 8)
+    qg = q.clone().requires_grad_(True)
+    kw_adj = dict(kw, q=qg)
+    fa = f.detach()
+    t0 = time.perf_counter()
+    loss = torch.zeros((), dtype=torch.float32)
+    for _ in range(n_adj):
+        fa, fp = O.hot_step(fa, **kw_adj)
+        loss = loss + O.momentum_exchange(fp, fa, qg)[0]
+    loss.backward()
+    dt_adj = time.perf_counter() - t0
+    assert qg.grad is not None and bool(torch.isfinite(qg.grad).all())
     return {"value": round(mlups, 3), "unit": "MLUPS", "cores": cores, "kind": "port",
             "one_thread_mlups": round(NX * NY * NZ * n1 / dt1 / 1e6, 3),
+            "fwd_adj_mlups": round(NX * NY * NZ * n_adj / dt_adj / 1e6, 3),
             "sample": f"{sample_steps} steps of the 256x128x128 MRT+Bouzidi sphere step incl. momentum exchange "
                       f"(oracle, torch CPU float32, {cores} threads) after 1 warm-up step: {dt:.2f} s; "
-                      f"{n1} step(s) on 1 thread: {dt1:.2f} s"}
+                      f"{n1} step(s) on 1 thread: {dt1:.2f} s; forward + torch autograd of sum F_x with "
+                      f"respect to q over {n_adj} steps ({cores} threads): {dt_adj:.2f} s"}
 
 
 def main():

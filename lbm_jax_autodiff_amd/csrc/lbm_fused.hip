@@ -28,8 +28,12 @@
 
 namespace lbm {
 
+// Threads per workgroup of a step launch.  256 for every sweep: the float32
+// sweep of 256 x 128 x 128 runs 2% faster than with 512 (108.6 against
+// 110.6 us per step, 384 x 192 x 192 380 against 384 us) since the force
+// reduction left the link workgroups' collision path, the 16-bit sweep is 7%
+// slower at 512, and small grids lose parallelism with larger workgroups.
 constexpr int kStepBlock = 256;
-constexpr int kStepBlockLarge = 512;
 
 template <typename R>
 struct StepArgs {
@@ -749,31 +753,14 @@ inline int choose_vec(const lbm_step_desc* d, uint32_t pstride, uint32_t rstride
   return 1;
 }
 
-// Threads per workgroup of a step launch: 512 for the one-cell-per-thread
-// float32 sweep of grids of 2^21 cells or more (1% faster than 256 at
-// 256 x 128 x 128; 768 and 1024 are slower), 256 otherwise (the 16-bit sweep
-// is 7% slower at 512, and small grids lose parallelism: 96 x 48 x 48 went
-// from 14.4 to 18.4 us per step).
-inline int step_block(const lbm_step_desc* d) {
-  long long ps = 0, rs = 0;
-  if (!state_layout(d, ps, rs)) return kStepBlock;
-  const long long cells = 1LL * d->nx * d->ny * d->nz;
-  const bool one_cell = choose_vec(d, uint32_t(ps), uint32_t(rs)) == 1;
-  return (d->storage == LBM_F32 && one_cell && cells >= (1LL << 21)) ? kStepBlockLarge : kStepBlock;
-}
-
 template <class S, int COLL, int MODE>
-int launch_step_cfg(const StepArgs<typename S::R>& a, int V, int bs, const typename S::T* in, typename S::T* out,
+int launch_step_cfg(const StepArgs<typename S::R>& a, int V, const typename S::T* in, typename S::T* out,
                     double* fpart, hipStream_t s) {
   const unsigned grid = unsigned(a.nlb + a.ndb);
-  constexpr bool f32 = std::is_same<S, StoreF32>::value;
   if (V == 4) hipLaunchKernelGGL((k_step<S, COLL, MODE, 4>), grid, kStepBlock, 0, s, a, in, out, fpart);
   else if (V == 2) hipLaunchKernelGGL((k_step<S, COLL, MODE, 2>), grid, kStepBlock, 0, s, a, in, out, fpart);
   else if (MODE != LBM_MODE_COLLIDE && a.row_waves)
     hipLaunchKernelGGL((k_step<S, COLL, MODE, 0>), grid, kStepBlock, 0, s, a, in, out, fpart);
-  else if (f32 && bs == kStepBlockLarge)
-    hipLaunchKernelGGL((k_step<S, COLL, MODE, 1, f32 ? kStepBlockLarge : kStepBlock>), grid, kStepBlockLarge, 0, s,
-                       a, in, out, fpart);
   else hipLaunchKernelGGL((k_step<S, COLL, MODE, 1>), grid, kStepBlock, 0, s, a, in, out, fpart);
   return hipGetLastError() == hipSuccess ? LBM_OK : LBM_E_LAUNCH;
 }
@@ -802,23 +789,22 @@ int launch_step(const lbm_step_desc* d, int mode, const void* src, void* dst, do
                    bytes < (1LL << 32)) ? 1 : 0;
     a.state_bytes = a.row_waves ? uint32_t(bytes) : 0u;
   }
-  const int bs = step_block(d);
   a.nlb = (mode == LBM_MODE_COLLIDE) ? 0 : (a.n_links + kLinkCellsPerBlock - 1) / kLinkCellsPerBlock;
-  const long long cells_per_block = 1LL * bs * V;
+  const long long cells_per_block = 1LL * kStepBlock * V;
   a.ndb = int((1LL * a.N + cells_per_block - 1) / cells_per_block);
   const auto* in = static_cast<const typename S::T*>(src);
   auto* out = static_cast<typename S::T*>(dst);
   const bool mrt = a.coll == LBM_COLL_MRT;
   switch (mode) {
     case LBM_MODE_FULL:
-      return mrt ? launch_step_cfg<S, LBM_COLL_MRT, LBM_MODE_FULL>(a, V, bs, in, out, fpart, s)
-                 : launch_step_cfg<S, LBM_COLL_BGK, LBM_MODE_FULL>(a, V, bs, in, out, fpart, s);
+      return mrt ? launch_step_cfg<S, LBM_COLL_MRT, LBM_MODE_FULL>(a, V, in, out, fpart, s)
+                 : launch_step_cfg<S, LBM_COLL_BGK, LBM_MODE_FULL>(a, V, in, out, fpart, s);
     case LBM_MODE_COLLIDE:
-      return mrt ? launch_step_cfg<S, LBM_COLL_MRT, LBM_MODE_COLLIDE>(a, V, bs, in, out, fpart, s)
-                 : launch_step_cfg<S, LBM_COLL_BGK, LBM_MODE_COLLIDE>(a, V, bs, in, out, fpart, s);
+      return mrt ? launch_step_cfg<S, LBM_COLL_MRT, LBM_MODE_COLLIDE>(a, V, in, out, fpart, s)
+                 : launch_step_cfg<S, LBM_COLL_BGK, LBM_MODE_COLLIDE>(a, V, in, out, fpart, s);
     case LBM_MODE_STREAM:
-      return mrt ? launch_step_cfg<S, LBM_COLL_MRT, LBM_MODE_STREAM>(a, V, bs, in, out, fpart, s)
-                 : launch_step_cfg<S, LBM_COLL_BGK, LBM_MODE_STREAM>(a, V, bs, in, out, fpart, s);
+      return mrt ? launch_step_cfg<S, LBM_COLL_MRT, LBM_MODE_STREAM>(a, V, in, out, fpart, s)
+                 : launch_step_cfg<S, LBM_COLL_BGK, LBM_MODE_STREAM>(a, V, in, out, fpart, s);
   }
   return LBM_E_ARG;
 }
