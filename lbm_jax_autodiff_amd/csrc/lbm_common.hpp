@@ -270,13 +270,16 @@ struct StoreHalfDev {
   // v_cvt_sr_f16_f32 / v_cvt_sr_bf16_f32: the 16-bit value above or below d,
   // the upper one with probability (d - lo) / (hi - lo), decided by the
   // random bits rbits (binary16 subnormals included).
+  // The conversion writes the low half of its destination and keeps the
+  // high half, which is never stored: d's own register serves as the old
+  // value, so no register has to be zeroed first.
   __device__ static unsigned short enc_sr(float v, float off, uint32_t rbits) {
     const float d = v - off;
     if constexpr (BF) {
-      const bf16x2 h = __builtin_amdgcn_cvt_sr_bf16_f32(bf16x2{}, d, rbits, false);
+      const bf16x2 h = __builtin_amdgcn_cvt_sr_bf16_f32(__builtin_bit_cast(bf16x2, d), d, rbits, false);
       return __builtin_bit_cast(u16x2, h)[0];
     } else {
-      const f16x2 h = __builtin_amdgcn_cvt_sr_f16_f32(f16x2{}, d, rbits, false);
+      const f16x2 h = __builtin_amdgcn_cvt_sr_f16_f32(__builtin_bit_cast(f16x2, d), d, rbits, false);
       return __builtin_bit_cast(u16x2, h)[0];
     }
   }
