@@ -387,7 +387,7 @@ __device__ __forceinline__ void adj_link_pull_part(const AdjArgs<typename S::R>&
       const bool high = qo[K] >= R(0.5);
       const size_t normal = i * N + (x_fallback<i>(x, a.nx) ? o : owner_target<i>(a, x, y, z));
       const size_t ka = low ? io * N + next_cell<i>(a, x, y, z) : (high ? i * N + o : normal);
-      va[K] = S::ld_c(P, ka, low ? a.off[io] : a.off[i]);
+      va[K] = S::ld_cv(P, ka, low ? a.off[io] : a.off[i]);
       vb[K] = S::ld_c(P, io * N + o, a.off[io]);
       if (a.add_prev) lg[K] = LM::rt(lg[K] + a.add_prev[size_t(i) * a.n_links + b], a.lam_scale, a.lam_inv);
     }

@@ -140,14 +140,16 @@ __device__ __forceinline__ unsigned short float_to_bf16_bits(float d) {
 __device__ __forceinline__ float f16_bits_to_float(unsigned short raw) { return __half2float(__ushort_as_half(raw)); }
 __device__ __forceinline__ unsigned short float_to_f16_bits(float d) { return __half_as_ushort(__float2half_rn(d)); }
 // float(h) + off for binary16 bits h, in one v_fma_mix_f32 (op_sel_hi marks
-// the half operand).  `off` must be wave-uniform (it is read from an SGPR).
+// the half operand).  For a wave-uniform `off` (every caller but the link
+// paths' per-lane offsets, which use ld_cv) the operand is a scalar register.
 __device__ __forceinline__ float f16_bits_add(unsigned short raw, float off) {
   float r;
   asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(uint32_t(raw)), "s"(off));
   return r;
 }
 // The binary16 bits of v - off, rounded once to nearest even, in one
-// v_fma_mixlo_f16.  `off` must be wave-uniform (it is read from an SGPR).
+// v_fma_mixlo_f16.  `off` is wave-uniform (a scalar register) at every
+// caller.
 __device__ __forceinline__ unsigned short f16_bits_of_diff(float v, float off) {
   uint32_t r;
   asm("v_fma_mixlo_f16 %0, %1, 1.0, -%2 op_sel_hi:[0,0,0]" : "=v"(r) : "v"(v), "s"(off));
@@ -172,6 +174,7 @@ struct StoreF32 {
   static constexpr int tag = LBM_F32;
   __device__ static R ld(const T* p, size_t k, R) { return __builtin_nontemporal_load(p + k); }
   __device__ static R ld_c(const T* p, size_t k, R) { return p[k]; }
+  __device__ static R ld_cv(const T* p, size_t k, R) { return p[k]; }
   __device__ static void st(T* p, size_t k, R v, R) { __builtin_nontemporal_store(v, p + k); }
   __device__ static R bld(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, R) {
     return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, int(vo), int(so), 0));
@@ -210,6 +213,7 @@ struct StoreF64 {
   static constexpr int tag = LBM_F64;
   __device__ static R ld(const T* p, size_t k, R) { return __builtin_nontemporal_load(p + k); }
   __device__ static R ld_c(const T* p, size_t k, R) { return p[k]; }
+  __device__ static R ld_cv(const T* p, size_t k, R) { return p[k]; }
   __device__ static void st(T* p, size_t k, R v, R) { __builtin_nontemporal_store(v, p + k); }
   __device__ static R bld(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, R) {
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, int(vo), int(so), 0));
@@ -241,8 +245,9 @@ struct StoreF64 {
     if constexpr (V == 1) __builtin_nontemporal_store(o[0], p + k);
   }
 };
-// Every access of a half format takes a wave-uniform offset (the offset of a
-// compile-time population, or of a population uniform over the workgroup).
+// Every access of a half format but ld_cv takes a wave-uniform offset (the
+// offset of a compile-time population, or of a population uniform over the
+// workgroup).
 // For binary16 the conversion and the offset fold into one mixed-precision
 // FMA each way: v_fma_mix_f32 (half operand * 1.0 + off, rounded once to
 // float) on load and v_fma_mixlo_f16 (v * 1.0 - off, rounded once to half,
@@ -265,6 +270,12 @@ struct StoreHalfDev {
   __device__ static unsigned short* bits(T* p) { return reinterpret_cast<unsigned short*>(p); }
   __device__ static R ld(const T* p, size_t k, R off) { return dec(__builtin_nontemporal_load(bits(p) + k), off); }
   __device__ static R ld_c(const T* p, size_t k, R off) { return dec(bits(p)[k], off); }
+  // a load whose offset differs between lanes (the Bouzidi pulls of link
+  // cells pick the offset of one of two populations per lane)
+  __device__ static R ld_cv(const T* p, size_t k, R off) {
+    const unsigned short raw = bits(p)[k];
+    return (BF ? bf16_bits_to_float(raw) : f16_bits_to_float(raw)) + off;
+  }
   __device__ static void st(T* p, size_t k, R v, R off) { __builtin_nontemporal_store(enc(v, off), bits(p) + k); }
   // Stochastic rounding of the deviation d = v - off with gfx950's
   // v_cvt_sr_f16_f32 / v_cvt_sr_bf16_f32: the 16-bit value above or below d,

@@ -430,7 +430,7 @@ __device__ __forceinline__ void link_pull_part(const StepArgs<typename S::R>& a,
       const bool high = qo[K] >= R(0.5);
       const size_t ka = low ? io * N + wrap_next<i>(a, x, y, z)
                             : (high ? i * N + o : i * N + pull_src<i>(a, rows, o, x));
-      va[K] = S::ld_c(src, ka, low ? a.off[io] : a.off[i]);
+      va[K] = S::ld_cv(src, ka, low ? a.off[io] : a.off[i]);
       vb[K] = S::ld_c(src, io * N + o, a.off[io]);
     }
   });
