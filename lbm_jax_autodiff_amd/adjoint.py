@@ -397,6 +397,8 @@ def rollout(f0: torch.Tensor, n_steps: int, *, tau, q: Optional[torch.Tensor] = 
                link_additions=link_additions, rounding=rounding, seed=int(seed))
     if link_additions not in ("fold", "scatter"):
         raise ValueError("link_additions must be 'fold' or 'scatter'")
+    if rounding not in ("nearest", "stochastic"):
+        raise ValueError("rounding must be 'nearest' or 'stochastic'")
     if int(n_steps) < 1:
         raise ValueError("n_steps must be >= 1")
     return _RolloutFn.apply(f0, q, porosity, u, cfg)

@@ -36,3 +36,15 @@ def test_require_shape_messages():
         F.require_shape(torch.zeros(3), "solid", (3,))
     assert F.same_device("cpu", torch.device("cpu"))
     assert not F.same_device("cpu", "cuda:0")
+
+
+def test_lattice_rejects_unknown_rounding():
+    with pytest.raises(ValueError, match="rounding"):
+        Lattice(NX, NY, NZ, tau=0.6, storage="f16", rounding="up", device="cuda:0")
+
+
+def test_rollout_rejects_unknown_rounding_before_any_launch():
+    from lbm_jax_autodiff_amd import adjoint as A
+    f = torch.zeros((19, NX, NY, NZ))
+    with pytest.raises(ValueError, match="rounding"):
+        A.rollout(f, 3, tau=0.6, storage="f16", rounding="up")
