@@ -122,3 +122,50 @@ def test_c3_dcd_dr(dev, storage, tol):
     print(f"c3 {storage}: loss {res['loss']:.9f} (oracle {g['loss']:.9f}, rel {rel_l:.2e}); dCd/dr "
           f"{res['dloss_dr_world']:.9f} (oracle autograd f64 {g['dloss_dr_world']:.9f}, rel {rel_g:.2e})")
     assert rel_l <= tol and rel_g <= tol, (rel_l, rel_g)
+
+
+def _c3(radius, storage):
+    nx, ny, nz = 96, 48, 48
+    torch.cuda.empty_cache()
+    return P.sphere_loss_and_grad(nx, ny, nz, 2000, P.world_center(nx, ny, nz, (24, 24, 24)), radius, avg_last=600,
+                                  storage=storage)
+
+
+def test_c3_adjoint_matches_central_differences(dev):
+    """The float64 adjoint against central differences of the same kernels'
+    loss: second order in h (2.1e-4 relative at h = 1e-3, 2e-8 at 1e-5), so
+    the gradient carries no missing or extra term."""
+    ad = _c3(0.5, "f64")["dloss_dr_world"]
+    fd = {}
+    for h in (1e-3, 1e-5):
+        fd[h] = (_c3(0.5 + h, "f64")["loss"] - _c3(0.5 - h, "f64")["loss"]) / (2.0 * h)
+    e3, e5 = abs(fd[1e-3] / ad - 1.0), abs(fd[1e-5] / ad - 1.0)
+    print(f"c3 f64 adjoint {ad:.12f}; FD h=1e-3 {fd[1e-3]:.12f} ({e3:.2e}), h=1e-5 {fd[1e-5]:.12f} ({e5:.2e})")
+    assert e5 < 1e-6
+    assert 1e-5 < e3 < 1e-3  # the O(h^2) truncation error is visible at h = 1e-3
+
+
+def test_c3_adam_twenty_iterations(dev):
+    """The C3 optimisation: 20 Adam iterations on r (lr 0.05, clipped to
+    [0.3, 0.8]) from r = 0.5 with float32 gradients.  The gradient stays
+    positive, so r falls by about lr per iteration to the lower bound, which
+    it reaches at iteration 4 and keeps; the loss at r = 0.5 is the C3 loss."""
+    seen = []
+
+    def lg(r):
+        out = _c3(r, "f32")
+        seen.append((r, out["loss"], out["dloss_dr_world"]))
+        return out["loss"], out["dloss_dr_world"]
+
+    r, hist = P.adam(lg, 0.5, 20, 0.05)
+    print("c3 adam:", [(round(h["r"], 6), round(h["loss"], 6), round(h["grad"], 4)) for h in hist])
+    assert len(hist) == 20 and len(seen) == 20
+    assert all(h["grad"] > 0.0 for h in hist)
+    assert abs(hist[0]["loss"] - 1.2723378) < 1e-6
+    # Adam's first step is lr times the gradient's sign
+    assert abs(hist[1]["r"] - 0.45) < 1e-6
+    first = min(t for t, h in enumerate(hist) if h["r"] == 0.3)
+    assert first in (4, 5)
+    assert all(0.04 < hist[t]["r"] - hist[t + 1]["r"] < 0.06 for t in range(first - 1))
+    assert all(h["r"] == 0.3 for h in hist[first:]) and r == 0.3
+    assert hist[-1]["loss"] > hist[0]["loss"]
