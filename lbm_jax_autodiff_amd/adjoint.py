@@ -141,7 +141,7 @@ class _Checkpoints:
         """{t: P_t} for t = t0 and every t0 < t < t1 with t % keep_every == 0, from P_t0."""
         lat = self.lat
         if keep_every == 1:
-            bufs = [p_start] + [lat.new_state() for _ in range(t1 - 1 - t0)]
+            bufs = [p_start] + lat.new_states(t1 - 1 - t0)
             lat.chain(bufs, step=t0)
             return {t0 + k: b for k, b in enumerate(bufs)}
         out = {t0: p_start}
@@ -204,8 +204,8 @@ class _RolloutFn(torch.autograd.Function):
         lat.collide(s0, a)
         fp = fpart if lat.slots else None
         if ck.store_all:
-            # every P_t in a buffer of its own
-            bufs = [a] + [lat.new_state() for _ in range(n_steps - 1)]
+            # every P_t in a buffer of its own, all carved from one allocation
+            bufs = [a] + lat.new_states(n_steps - 1)
             lat.chain(bufs, fp, step=0)
             for t, buf in enumerate(bufs):
                 ck.record(t, buf)

@@ -29,6 +29,9 @@ class _CountingLattice:
     def new_state(self):
         return torch.zeros(1)
 
+    def new_states(self, n):
+        return [self.new_state() for _ in range(n)]
+
     def clone_state(self, buf):
         return buf.clone()
 
