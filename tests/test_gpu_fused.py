@@ -170,6 +170,38 @@ def test_state_layout_does_not_change_results(dev, storage, vec):
             assert torch.equal(a, b)
 
 
+@pytest.mark.parametrize("storage,layout", [("f16", "rows"), ("f32", "apart"), ("f64", "rows")])
+def test_states_from_one_allocation(dev, storage, layout):
+    """Lattice.new_states carves n state buffers from one allocation: each
+    starts on a 256-byte boundary, none overlaps another, and a chain through
+    them gives the same states and forces as separately allocated buffers."""
+    nx, ny, nz = 13, 7, 5  # an extent that is not a multiple of 256 bytes
+    f0 = near_eq_state(nx, ny, nz, amp=2e-3)
+    q = random_q(nx, ny, nz, seed=4, p_wall=0.1)
+    dt = torch.float64 if storage == "f64" else torch.float32
+    lat = Lattice(nx, ny, nz, tau=0.6, collision="mrt", q=q.to(dev), guard="pre", storage=storage, device=dev,
+                  layout=layout)
+    n = 6
+    bufs = lat.new_states(n)
+    esize = bufs[0].element_size()
+    spans = []
+    for b in bufs:
+        assert b.data_ptr() % 256 == 0
+        assert b.shape == (19, nz, ny, nx) and b.stride() == (lat.P, ny * lat.RS, lat.RS, 1)
+        spans.append((b.data_ptr(), b.data_ptr() + lat.extent * esize))
+    spans.sort()
+    assert all(e0 <= s1 for (_, e0), (s1, _) in zip(spans, spans[1:]))
+    s0 = lat.to_state(f0.to(dt).to(dev))
+    out = []
+    for states in (bufs, [lat.new_state() for _ in range(n)]):
+        states[0].copy_(s0)
+        fp = torch.zeros((n - 1, max(lat.slots, 1), 3), dtype=torch.float64, device=dev)
+        lat.chain(states, fp if lat.slots else None, step=0)
+        out.append([cpu(lat.from_state(states[-1])), cpu(fp)])
+    for a, b in zip(*out):
+        assert torch.equal(a, b)
+
+
 ROW_CASES = list(itertools.product(["f16", "bf16"], ["bgk", "mrt"], ["none", "hard", "soft"],
                                    ["none", "post", "pre"], [True, False]))
 
