@@ -240,10 +240,12 @@ class Lattice:
     def new_states(self, n: int, dtype: Optional[torch.dtype] = None) -> list:
         """``n`` uninitialised state buffers carved from one allocation, each
         starting on a 256-byte boundary (one allocator call instead of n)."""
+        if n <= 0:
+            return []
         dt = dtype or self.buf_dtype
         esize = torch.empty((), dtype=dt).element_size()
         step = -(-self.extent * esize // 256) * 256 // esize
-        flat = torch.empty(max(n, 1) * step, dtype=dt, device=self.device)
+        flat = torch.empty(n * step, dtype=dt, device=self.device)
         shape = (19, self.nz, self.ny, self.nx)
         strides = (self.P, self.ny * self.RS, self.RS, 1)
         return [flat.as_strided(shape, strides, k * step) for k in range(n)]
