@@ -8,6 +8,8 @@
 // conventions are JAX's transposition of the reference graph: `where` routes
 // the cotangent to the taken branch, `clip` passes it inside its bounds only,
 // comparisons carry none.
+#include <cmath>
+
 #include "lbm_adjoint_physics.hpp"
 #include "lbm_geometry.hpp"
 
@@ -419,14 +421,19 @@ __global__ void k_drag_vjp(const double* __restrict__ force, const double* __res
 //     dt/dc = (2 d + (2 b d - 4 a o) / sqrt(disc)) / (2a + eps),
 //   only where the ray hit is valid, disc > floor and 0.01 < t < 0.99
 //   (the where, max and clip then pass the cotangent), else 0.
+// The sweep covers the box [box[0], box[1]) x [box[2], box[3]) x [box[4],
+// box[5]) of the grid: all of it when the porosity term is summed, else the
+// sphere's surroundings, the only cells with links.
 template <typename C, typename O>
 __global__ void k_sphere_vjp(int mode, C cx, C cy, C cz, C r, C alpha, C disc_floor, C denom_eps,
                              const int32_t* __restrict__ solid, const O* __restrict__ gphi, const O* __restrict__ gq,
-                             double* __restrict__ part, int nx, int ny, int nz) {
+                             double* __restrict__ part, int nx, int ny, int nz, int4 lo, int4 ext) {
   const size_t N = size_t(nx) * ny * nz;
+  const size_t M = size_t(ext.x) * ext.y * ext.z;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (size_t c = blockIdx.x * size_t(blockDim.x) + threadIdx.x; c < N; c += size_t(gridDim.x) * blockDim.x) {
-    const int x = int(c % nx), y = int((c / nx) % ny), z = int(c / (size_t(nx) * ny));
+  for (size_t k = blockIdx.x * size_t(blockDim.x) + threadIdx.x; k < M; k += size_t(gridDim.x) * blockDim.x) {
+    const int x = lo.x + int(k % ext.x), y = lo.y + int((k / ext.x) % ext.y), z = lo.z + int(k / (size_t(ext.x) * ext.y));
+    const size_t c = (size_t(z) * ny + y) * nx + x;
     C wx, wy, wz;
     world_xyz(x, y, z, nx, ny, nz, wx, wy, wz);
     const C ox = wx - cx, oy = wy - cy, oz = wz - cz;
@@ -479,15 +486,32 @@ __global__ void k_sphere_vjp(int mode, C cx, C cy, C cz, C r, C alpha, C disc_fl
 }
 
 // out[k] = sum over the rows of part[rows][4], one thread per k, fixed order.
-__global__ void k_sum4(const double* __restrict__ part, int rows, double* __restrict__ out) {
-  const int k = threadIdx.x;
-  if (k >= 4) return;
-  double s = 0.0;
-  for (int r = 0; r < rows; ++r) s += part[r * 4 + k];
-  out[k] = s;
-}
-
 constexpr int kGeomVjpBlocks = 512;
+constexpr int kSum4Threads = 256;
+
+// out[k] = sum over the kGeomVjpBlocks rows of part[rows][4] in a fixed
+// order: each thread adds its rows t, t + 256, ..., then a halving tree in LDS.
+__global__ __launch_bounds__(kSum4Threads) void k_sum4(const double* __restrict__ part, double* __restrict__ out) {
+  static_assert(kGeomVjpBlocks % kSum4Threads == 0, "whole rows per thread");
+  __shared__ double sh[4][kSum4Threads];
+  const int t = threadIdx.x;
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int r = t; r < kGeomVjpBlocks; r += kSum4Threads) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) s[k] += part[r * 4 + k];
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) sh[k][t] = s[k];
+  __syncthreads();
+  for (int h = kSum4Threads / 2; h > 0; h /= 2) {
+    if (t < h) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) sh[k][t] += sh[k][t + h];
+    }
+    __syncthreads();
+  }
+  if (t < 4) out[t] = sh[t][0];
+}
 
 }  // namespace lbm
 
@@ -641,17 +665,36 @@ int lbm_sphere_geometry_vjp(int mode, int coord_dtype, int out_dtype, double cx,
                             int nz, lbm_stream_t stream) {
   if (!vjp_dims_ok(nx, ny, nz) || !solid || !out || !workspace || (mode != 0 && mode != 1)) return LBM_E_ARG;
   hipStream_t s = (hipStream_t)stream;
+  // Without the porosity term only link cells contribute: fluid cells next
+  // to a body cell, and body cells lie within the radius of the centre.  The
+  // sweep then covers the sphere's box in lattice indices, node i at world
+  // (i + 1/2) / s - offset, widened by three cells for the link reach and
+  // the rounding of the masks.
+  int4 lo = make_int4(0, 0, 0, 0), ext = make_int4(nx, ny, nz, 0);
+  if (!(g_porosity && mode == 1) && std::isfinite(cx) && std::isfinite(cy) && std::isfinite(cz) &&
+      std::isfinite(radius)) {
+    auto span = [&](double c, double off, double sc, int n, int& l, int& e) {
+      const double ic = (c + off) * sc - 0.5, rr = std::fabs(radius) * sc;
+      const double a = std::fmin(std::fmax(std::floor(ic - rr) - 3.0, 0.0), double(n));
+      const double b = std::fmin(std::fmax(std::ceil(ic + rr) + 4.0, a), double(n));
+      l = int(a);
+      e = int(b) - int(a);
+    };
+    span(cx, 4.0, nx / 8.0, nx, lo.x, ext.x);
+    span(cy, 2.0, ny / 4.0, ny, lo.y, ext.y);
+    span(cz, 2.0, nz / 4.0, nz, lo.z, ext.z);
+  }
 #define LBM_GEOM_VJP(C, O)                                                                                          \
   hipLaunchKernelGGL((k_sphere_vjp<C, O>), kGeomVjpBlocks, kVjpBlock, 0, s, mode, C(cx), C(cy), C(cz), C(radius),   \
                      C(sharpness), C(mode == 0 ? 0.0 : disc_floor), C(denom_eps), solid, (const O*)g_porosity,       \
-                     (const O*)g_q, workspace, nx, ny, nz)
+                     (const O*)g_q, workspace, nx, ny, nz, lo, ext)
   if (coord_dtype == LBM_F32 && out_dtype == LBM_F32) { LBM_GEOM_VJP(float, float); }
   else if (coord_dtype == LBM_F32 && out_dtype == LBM_F64) { LBM_GEOM_VJP(float, double); }
   else if (coord_dtype == LBM_F64 && out_dtype == LBM_F32) { LBM_GEOM_VJP(double, float); }
   else if (coord_dtype == LBM_F64 && out_dtype == LBM_F64) { LBM_GEOM_VJP(double, double); }
   else return LBM_E_DTYPE;
 #undef LBM_GEOM_VJP
-  hipLaunchKernelGGL(k_sum4, 1, 64, 0, s, workspace, kGeomVjpBlocks, out);
+  hipLaunchKernelGGL(k_sum4, 1, kSum4Threads, 0, s, workspace, out);
   return vjp_status();
 }
 

@@ -129,6 +129,9 @@ class _SphereGeometry(torch.autograd.Function):
     @staticmethod
     def forward(ctx, radius, center, nx, ny, nz, mode, sharpness, coord_dtype, out_dtype, disc_floor, denom_eps,
                 device):
+        # an output the loss does not use arrives as None, not as zeros: without
+        # a porosity cotangent the backward sweeps only the sphere's box
+        ctx.set_materialize_grads(False)
         c = _center_floats(center)
         solid, por, q, _, _ = sphere_geometry_raw(nx, ny, nz, c, radius, mode=mode, sharpness=sharpness,
                                                   coord_dtype=coord_dtype, out_dtype=out_dtype,

@@ -310,6 +310,37 @@ def test_smooth_sphere_center_and_radius_gradients(dev, center):
         assert abs(a - b) <= 1e-10 * max(1.0, abs(b)), (gc, gc_ref)
 
 
+@pytest.mark.parametrize("center,radius", [((-0.3, 0.1, -0.05), 0.55), ((-3.7, 1.6, -1.8), 0.5),
+                                           ((4.2, 0.0, 0.0), 0.6), ((0.0, -1.9, 1.95), 1.1)])
+def test_sphere_q_gradient_sweeps_only_the_sphere_box(dev, center, radius):
+    """With no porosity cotangent the geometry's reverse product sweeps only
+    the sphere's box of cells; a zero porosity cotangent makes it sweep the
+    whole grid.  Both give dq/d(r, c) of the oracle, for spheres inside the
+    domain, against and across its faces (clamped boxes)."""
+    nx, ny, nz = 32, 16, 16
+    g = torch.Generator().manual_seed(14)
+    w = torch.randn((19, nx, ny, nz), generator=g, dtype=DT)
+
+    def grads(geom, device, with_por):
+        c = torch.tensor(center, dtype=DT, device=device, requires_grad=True)
+        r = torch.tensor(radius, dtype=DT, device=device, requires_grad=True)
+        por, _, q = geom(c, r)
+        loss = (q * w.to(q.device)).sum()
+        if with_por:
+            loss = loss + 0.0 * por.sum()
+        gr, gc = torch.autograd.grad(loss, (r, c))
+        return [float(gr)] + [float(v) for v in gc]
+
+    ref = grads(lambda c, r: O.smooth_sphere_geometry(nx, ny, nz, c, r, 10.0, coord_dtype=DT), "cpu", False)
+    box = grads(lambda c, r: lbm.smooth_sphere_geometry(nx, ny, nz, c, r, 10.0, coord_dtype=DT, out_dtype=DT),
+                dev, False)
+    full = grads(lambda c, r: lbm.smooth_sphere_geometry(nx, ny, nz, c, r, 10.0, coord_dtype=DT, out_dtype=DT),
+                 dev, True)
+    scale = max(1.0, max(abs(v) for v in ref))
+    assert max(abs(a - b) for a, b in zip(box, ref)) <= 1e-10 * scale, (box, ref)
+    assert max(abs(a - b) for a, b in zip(box, full)) <= 1e-12 * scale, (box, full)
+
+
 def test_create_sphere_q_gradients(dev):
     """The hard sphere's ray q (primitives.py:56-174) in radius and centre,
     float32 coordinates as in the reference, against oracle autograd."""
