@@ -570,10 +570,10 @@ __device__ __forceinline__ void adjoint_row(const AdjArgs<typename S::R>& a, con
     const uint32_t e = uint32_t(i) * a.P + row_of(EY[i], EZ[i]);
     s[i] = S::bld(rp, xs * B, e * B, a.off[i]);
   });
-  // 16-bit sweeps (bound by vector issue as much as by memory) load the
-  // cotangents together with the pulled values, so a wave waits for one
-  // round of loads rather than two (FP16 forward + adjoint at 256 x 128 x 128:
-  // 158 -> 153 us per step; no gain in FP32)
+  // 16-bit sweeps load the cotangents together with the pulled values, so a
+  // wave waits for one round of loads rather than two (FP16 forward + adjoint
+  // at 256 x 128 x 128: 158 -> 153 us per step); the float32 sweep was 1%
+  // slower this way
   constexpr bool early = sizeof(typename S::T) == 2;
   if constexpr (early) {
     sfor<Q>([&](auto I) {
