@@ -780,8 +780,8 @@ int launch_step(const lbm_step_desc* d, int mode, const void* src, void* dst, do
   const int V = choose_vec(d, a.P, a.RS);
   {
     // The row-uniform sweep (32-bit byte offsets) for 2-byte populations,
-    // whose sweep is bound by vector instructions: 62.2 -> 57.4 us per step
-    // at 256 x 128 x 128.  The float32 sweep is bound by memory and runs
+    // whose per-lane-addressed sweep was bound by vector instructions:
+    // 62.2 -> 57.4 us per step at 256 x 128 x 128.  The float32 sweep is bound by memory and runs
     // 0.5% faster on the per-lane addresses, so it keeps them.
     const long long extent = 1LL * (Q - 1) * a.P + (1LL * a.ny * a.nz - 1) * a.RS + a.nx;
     const long long bytes = extent * (long long)sizeof(typename S::T);
