@@ -389,9 +389,12 @@ __device__ __forceinline__ void dense_row(const StepArgs<typename S::R>& a, cons
   });
 }
 
-// Register budget of the step kernel at one cell per thread: 80 VGPRs (6
-// waves per SIMD) with FP32 arithmetic, which the dense sweep (about 60) and
-// the link path (about 77) fit without spills; 168 (3 waves) with FP64.
+// Register budget of the step kernel at one cell per thread: at most 80 VGPRs
+// (at least 6 waves per SIMD) with float arithmetic, 168 (3 waves) with FP64.
+// The float32 step takes 61 VGPRs and runs 8 waves per SIMD; the 16-bit step
+// 61 VGPRs and 103 SGPRs (its scalar row offsets), so 7 waves.  Asking for 8
+// fits it in 78 SGPRs with lane spills and changed nothing (57.0 us either
+// way at 256 x 128 x 128).
 template <class S, int V>
 constexpr int step_waves_per_eu() {
   return V > 1 ? 1 : (sizeof(typename S::R) == 8 ? 3 : 6);
