@@ -54,17 +54,40 @@ class LBMParams(NamedTuple):
     periodic_yz: bool = True
 
 
+# The lattice of the last forward step/simulate call, so that a loop of
+# step() calls on one geometry compiles its link list once.  The entry holds
+# the solid and q tensors themselves and their versions: it serves only those
+# objects (never a later tensor at a reused address) and only unchanged (an
+# in-place write bumps the version and rebuilds it).
+_last_lattice = None
+
+
+def _version(t):
+    return None if t is None else t._version
+
+
 def _lattice(state: LBMState, params: LBMParams) -> Lattice:
+    global _last_lattice
     nx, ny, nz = F.dims_of_pop(state.f)
     vel = params.inlet_velocity
     if isinstance(vel, torch.Tensor):
         vel = vel.detach().reshape(-1).to("cpu", torch.float64).tolist()
+    vel = tuple(float(v) for v in vel)
     storage = "f64" if state.f.dtype == torch.float64 else "f32"
-    return Lattice(nx, ny, nz, tau=F.scalar_float(params.tau), inlet_velocity=vel,
-                   collision="mrt" if int(params.collision_mode) == 1 else "bgk",
-                   smagorinsky_cs=float(params.smagorinsky_cs) if int(params.use_smagorinsky) else None,
-                   solid=state.solid, q=state.q, guard="post", periodic_yz=bool(params.periodic_yz),
-                   storage=storage, device=state.f.device)
+    key = (nx, ny, nz, str(state.f.device), storage, F.scalar_float(params.tau), vel, int(params.collision_mode),
+           int(params.use_smagorinsky), float(params.smagorinsky_cs), bool(params.periodic_yz))
+    if _last_lattice is not None:
+        k, solid, q, versions, lat = _last_lattice
+        if (k == key and solid is state.solid and q is state.q
+                and versions == (_version(state.solid), _version(state.q))):
+            return lat
+    lat = Lattice(nx, ny, nz, tau=F.scalar_float(params.tau), inlet_velocity=vel,
+                  collision="mrt" if int(params.collision_mode) == 1 else "bgk",
+                  smagorinsky_cs=float(params.smagorinsky_cs) if int(params.use_smagorinsky) else None,
+                  solid=state.solid, q=state.q, guard="post", periodic_yz=bool(params.periodic_yz),
+                  storage=storage, device=state.f.device)
+    _last_lattice = (key, state.solid, state.q, (_version(state.solid), _version(state.q)), lat)
+    return lat
 
 
 def _tau_is_field(params: LBMParams) -> bool:

@@ -202,6 +202,43 @@ def test_states_from_one_allocation(dev, storage, layout):
         assert torch.equal(a, b)
 
 
+def test_step_loop_reuses_the_lattice_until_the_geometry_changes(dev):
+    """A loop of step() calls compiles the link list once and equals
+    simulate() bit for bit; an in-place write to q, or another q tensor,
+    builds a new lattice whose result equals a freshly built one."""
+    import lbm_jax_autodiff_amd as lbm
+    from lbm_jax_autodiff_amd import state as state_mod
+    nx, ny, nz = 32, 16, 16
+    f0 = near_eq_state(nx, ny, nz, amp=2e-3).to(dev, torch.float32)
+    q = random_q(nx, ny, nz, seed=9, p_wall=0.05).to(dev, torch.float32)
+    g = torch.Generator().manual_seed(3)
+    solid = (torch.rand((nx, ny, nz), generator=g) < 0.05).to(torch.int32).to(dev)
+    params = lbm.LBMParams(0.6, (0.05, 0.0, 0.0), 1)
+    st = lbm.LBMState(f0, solid, q)
+    st = lbm.step(st, params)
+    lat = state_mod._last_lattice[-1]
+    for _ in range(4):
+        st = lbm.step(st, params)
+        assert state_mod._last_lattice[-1] is lat
+    ref = lbm.simulate(lbm.LBMState(f0, solid, q), params, 5)
+    assert torch.equal(st.f, ref.f)
+
+    def fresh(qq):
+        lt = Lattice(nx, ny, nz, tau=0.6, inlet_velocity=(0.05, 0.0, 0.0), collision="mrt", solid=solid, q=qq,
+                     guard="post", device=dev)
+        return lt.run(f0, 1, forces=False)[0]
+
+    q[q >= 0] = 0.5  # in place: same tensor, new version
+    one = lbm.step(lbm.LBMState(f0, solid, q), params)
+    assert state_mod._last_lattice[-1] is not lat
+    assert torch.equal(one.f, fresh(q))
+    lat2 = state_mod._last_lattice[-1]
+    q2 = random_q(nx, ny, nz, seed=10, p_wall=0.05).to(dev, torch.float32)
+    two = lbm.step(lbm.LBMState(f0, solid, q2), params)
+    assert state_mod._last_lattice[-1] is not lat2
+    assert torch.equal(two.f, fresh(q2))
+
+
 ROW_CASES = list(itertools.product(["f16", "bf16"], ["bgk", "mrt"], ["none", "hard", "soft"],
                                    ["none", "post", "pre"], [True, False]))
 
