@@ -32,6 +32,7 @@ struct AdjArgs {
   uint32_t RS;  // elements between consecutive (y, z) rows (state and cotangent buffers)
   FastDiv dnx, dny;
   int smag, bounce, guard, n_links, nlb, ndb;
+  uint32_t tgroup;  // dense tiles per XCD group (xcd_group_tile)
   R tau, omega, cs;
   R uin[3];
   R rho_out;
@@ -624,8 +625,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(adj_waves_pe
     adjoint_link_block<S, LM, COLL, MODE, BS>(a, P, lam_in, lam_out, gF, bid);
     return;
   }
-  // tiles in workgroup order: consecutive rows run on all XCDs at once
-  const uint32_t tile = bid - uint32_t(a.nlb);
+  // tiles in workgroup order (grouped per XCD where a tile holds part of a
+  // row): consecutive rows run on all XCDs at once
+  const uint32_t tile = xcd_group_tile(bid - uint32_t(a.nlb), uint32_t(a.ndb), a.tgroup);
   const uint32_t c = tile * BS + threadIdx.x;
   if (c >= a.N) return;
   if constexpr (ROW) {
@@ -790,6 +792,7 @@ int make_adj_args(const lbm_step_desc* d, const lbm_adjoint_desc* ad, int mode, 
   if (a.n_links < 0 || (a.n_links > 0 && (!a.link_cell || !a.link_q || !a.flags || !a.mu_link))) return LBM_E_ARG;
   a.nlb = (mode == LBM_MODE_COLLIDE) ? 0 : (a.n_links + kLinkCellsPerBlock - 1) / kLinkCellsPerBlock;
   a.ndb = int((n + kAdjBlock - 1) / kAdjBlock);
+  a.tgroup = tile_group_size(uint32_t(a.nx), uint32_t(kAdjBlock));
   return LBM_OK;
 }
 
@@ -819,6 +822,7 @@ int launch_adjoint(const lbm_step_desc* d, const lbm_adjoint_desc* ad, int mode,
   if (large) {
     a.nlb = (a.n_links + kLinkCellsPerBlock - 1) / kLinkCellsPerBlock;
     a.ndb = int((1LL * a.N + kAdjBlockLarge - 1) / kAdjBlockLarge);
+    a.tgroup = tile_group_size(uint32_t(a.nx), uint32_t(kAdjBlockLarge));
   }
   a.state_bytes = row ? uint32_t(sb) : 0u;
   a.lam_bytes = row ? uint32_t(lb) : 0u;

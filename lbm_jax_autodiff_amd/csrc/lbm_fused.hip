@@ -43,6 +43,7 @@ struct StepArgs {
   uint32_t RS;  // elements between consecutive (y, z) rows of a state buffer
   FastDiv dnx, dny;
   int coll, smag, bounce, guard, periodic, n_links, nlb, ndb;
+  uint32_t tgroup;      // dense tiles per XCD group (xcd_group_tile)
   int row_waves;        // nx is a multiple of 64: each wave's cells share one (y, z) row
   uint32_t state_bytes;  // extent of a state buffer in bytes (below 2^32 when row_waves)
   int sround;            // 16-bit storage rounds stochastically (dither hash seeded by `seed`)
@@ -610,7 +611,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(step_waves_p
 
   // ---------------- dense sweep ----------------
   const uint32_t first = (MODE == LBM_MODE_COLLIDE) ? 0u : uint32_t(a.nlb);
-  const uint32_t tile = bid - first;
+  const uint32_t tile = xcd_group_tile(bid - first, uint32_t(a.ndb), a.tgroup);
   if constexpr (V > 1) {
     dense_vec<S, COLL, MODE, V>(a, src, dst, tile * BS + threadIdx.x);
     return;
@@ -731,6 +732,7 @@ int make_args(const lbm_step_desc* d, int mode, StepArgs<R>& a) {
   if (a.guard < 0 || a.guard > 2 || a.bounce < 0 || a.bounce > 2) return LBM_E_ARG;
   a.nlb = (mode == LBM_MODE_COLLIDE) ? 0 : (a.n_links + kLinkCellsPerBlock - 1) / kLinkCellsPerBlock;
   a.ndb = int((n + kStepBlock - 1) / kStepBlock);
+  a.tgroup = 1u;
   a.row_waves = 0;
   a.state_bytes = 0;
   a.sround = 0;
@@ -795,6 +797,7 @@ int launch_step(const lbm_step_desc* d, int mode, const void* src, void* dst, do
   a.nlb = (mode == LBM_MODE_COLLIDE) ? 0 : (a.n_links + kLinkCellsPerBlock - 1) / kLinkCellsPerBlock;
   const long long cells_per_block = 1LL * kStepBlock * V;
   a.ndb = int((1LL * a.N + cells_per_block - 1) / cells_per_block);
+  a.tgroup = tile_group_size(uint32_t(a.nx), uint32_t(cells_per_block));
   const auto* in = static_cast<const typename S::T*>(src);
   auto* out = static_cast<typename S::T*>(dst);
   const bool mrt = a.coll == LBM_COLL_MRT;
